@@ -1,0 +1,174 @@
+"""Python front end of the C ABI: plans, batches and the test-support kernels.
+
+Device memory is whatever the caller passes (an integer device address, e.g.
+torch.Tensor.data_ptr()); streams are hipStream_t handles as integers
+(torch.cuda.Stream.cuda_stream) or None for the null stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, G2GPlanInfo, DTYPE_SIZE
+
+
+def _scalar_bytes(dtype, value):
+    """One element of dtype as a ctypes buffer (None stays None)."""
+    if value is None:
+        return None
+    if dtype == _lib.FLOAT32:
+        arr = np.array([value], dtype=np.float32)
+    elif dtype == _lib.FLOAT64:
+        arr = np.array([value], dtype=np.float64)
+    elif dtype == _lib.COMPLEX64:
+        arr = np.array([value], dtype=np.complex64)
+    else:
+        arr = np.array([value], dtype=np.complex128)
+    return ctypes.create_string_buffer(arr.tobytes(), arr.nbytes)
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return ctypes.c_void_p(s.cuda_stream)
+    return ctypes.c_void_p(int(s))
+
+
+class Plan:
+    """grid2grid::transform(initial, final, trans, alpha, beta) planned once
+    for a layout pair (see g2g_plan_create)."""
+
+    def __init__(self, initial, init_base, final, final_base, dtype, trans="N", stream=None):
+        lib = _lib.load()
+        self.dtype = dtype
+        self.trans = trans
+        es = DTYPE_SIZE.get(dtype, 8)  # an unknown dtype is rejected by the library
+        self._di = initial.descriptor(init_base, es)
+        self._df = final.descriptor(final_base, es)
+        h = ctypes.c_void_p()
+        check(lib.g2g_plan_create(self._di.ref(), self._df.ref(), dtype, trans.encode(), _stream(stream),
+                                  ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().g2g_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def info(self):
+        i = G2GPlanInfo()
+        check(_lib.load().g2g_plan_get_info(self.handle, ctypes.byref(i)))
+        return i
+
+    def transform(self, alpha=None, beta=None, stream=None):
+        a = _scalar_bytes(self.dtype, alpha)
+        b = _scalar_bytes(self.dtype, beta)
+        check(_lib.load().g2g_transform(self.handle, a, b, _stream(stream)))
+
+    def transform_staged(self, alpha=None, beta=None, stream=None):
+        a = _scalar_bytes(self.dtype, alpha)
+        b = _scalar_bytes(self.dtype, beta)
+        check(_lib.load().g2g_transform_staged(self.handle, a, b, _stream(stream)))
+
+    def counts(self):
+        P = self.info.n_ranks
+        out = np.zeros(P * P, dtype=np.int64)
+        check(_lib.load().g2g_plan_counts(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                          P * P))
+        return out.reshape(P, P)
+
+    def wire(self):
+        n = self.info.n_pieces
+        i32 = lambda: np.zeros(n, dtype=np.int32)
+        i64 = lambda: np.zeros(n, dtype=np.int64)
+        out = dict(row0=i32(), col0=i32(), rows=i32(), cols=i32(), src=i32(), dst=i32(),
+                   send_off=i64(), recv_off=i64())
+        P32 = ctypes.POINTER(ctypes.c_int32)
+        P64 = ctypes.POINTER(ctypes.c_int64)
+        check(_lib.load().g2g_plan_wire(
+            self.handle, *(out[k].ctypes.data_as(P32) for k in ("row0", "col0", "rows", "cols", "src", "dst")),
+            out["send_off"].ctypes.data_as(P64), out["recv_off"].ctypes.data_as(P64), n))
+        return out
+
+    def send_buffer(self, rank):
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        check(_lib.load().g2g_plan_send_buffer(self.handle, rank, ctypes.byref(p), ctypes.byref(n)))
+        return p.value or 0, n.value
+
+
+class Batch:
+    """Several plans run by one persistent launch (g2g_batch_*)."""
+
+    def __init__(self, plans, alphas=None, betas=None, stream=None):
+        lib = _lib.load()
+        n = len(plans)
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * n)(*[p.handle for p in plans])
+        self._scal = []
+        a_arr = b_arr = None
+        if alphas is not None:
+            a_bufs = [_scalar_bytes(p.dtype, a) for p, a in zip(plans, alphas)]
+            b_bufs = [_scalar_bytes(p.dtype, b) for p, b in zip(plans, betas)]
+            self._scal = a_bufs + b_bufs
+            a_arr = (ctypes.c_void_p * n)(*[ctypes.cast(x, ctypes.c_void_p) if x is not None else None
+                                            for x in a_bufs])
+            b_arr = (ctypes.c_void_p * n)(*[ctypes.cast(x, ctypes.c_void_p) if x is not None else None
+                                            for x in b_bufs])
+        h = ctypes.c_void_p()
+        check(lib.g2g_batch_create(n, arr, a_arr, b_arr, _stream(stream), ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self, stream=None):
+        check(_lib.load().g2g_batch_execute(self.handle, _stream(stream)))
+
+    def execute_graph(self, stream):
+        check(_lib.load().g2g_batch_execute_graph(self.handle, _stream(stream)))
+
+    @property
+    def bytes(self):
+        v = ctypes.c_int64()
+        check(_lib.load().g2g_batch_bytes(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().g2g_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_tuning(tile_bytes=0, persistent=-1):
+    check(_lib.load().g2g_set_tuning(int(tile_bytes), int(persistent)))
+
+
+def fill_layout(layout, base, dtype, seed, stream=None):
+    d = layout.descriptor(base, DTYPE_SIZE[dtype])
+    check(_lib.load().g2g_fill_layout(d.ref(), dtype, ctypes.c_uint64(seed), _stream(stream)))
+
+
+def verify_layout(layout, base, dtype, seed, trans, a_n_cols, stream=None):
+    d = layout.descriptor(base, DTYPE_SIZE[dtype])
+    bad = ctypes.c_int64()
+    check(_lib.load().g2g_verify_layout(d.ref(), dtype, ctypes.c_uint64(seed), trans.encode(), a_n_cols,
+                                        _stream(stream), ctypes.byref(bad)))
+    return bad.value

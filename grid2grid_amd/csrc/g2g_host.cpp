@@ -1,0 +1,830 @@
+// Host side of the C ABI (include/grid2grid_amd.h): layout validation, the
+// grid cover, plan objects, launches of the planning and gather kernels,
+// batches and hipGraph replay.
+#include "grid2grid_amd.h"
+#include "g2g_internal.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cctype>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace g2g;
+
+// ------------------------------------------------------------- errors
+
+namespace {
+
+thread_local std::string t_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_last_error = buf;
+    return code;
+}
+
+#define G2G_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(G2G_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+#define G2G_TRY(call)            \
+    do {                         \
+        int rc_ = (call);        \
+        if (rc_ != G2G_OK) return rc_; \
+    } while (0)
+
+uint32_t dtype_size(int dtype) {
+    switch (dtype) {
+    case G2G_FLOAT32: return 4;
+    case G2G_FLOAT64: return 8;
+    case G2G_COMPLEX64: return 8;
+    case G2G_COMPLEX128: return 16;
+    default: return 0;
+    }
+}
+
+bool dtype_complex(int dtype) { return dtype == G2G_COMPLEX64 || dtype == G2G_COMPLEX128; }
+
+// Device allocation owned by an object; freed on destruction.
+template <typename T>
+struct dbuf {
+    T *p = nullptr;
+    size_t n = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(G2G_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        n = count;
+        return G2G_OK;
+    }
+    int upload(const T *h, size_t count, hipStream_t s) {
+        G2G_TRY(alloc(count));
+        if (count) G2G_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return G2G_OK;
+    }
+};
+
+// Tuning knobs (g2g_set_tuning).
+std::atomic<int64_t> g_tile_bytes{0};
+std::atomic<int> g_persistent{-1};
+
+int device_cus() {
+    static std::once_flag once;
+    static int cus = 256;
+    std::call_once(once, [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            cus = v;
+    });
+    return cus;
+}
+
+// Work-item size: large enough to amortise per-item setup, small enough to
+// give every CU several items.
+uint64_t choose_tile_bytes(uint64_t total) {
+    const int64_t forced = g_tile_bytes.load();
+    if (forced > 0) return (uint64_t)forced;
+    uint64_t tb = total / (uint64_t)(device_cus() * 8);
+    tb = std::max<uint64_t>(tb, 16384);
+    tb = std::min<uint64_t>(tb, 262144);
+    return (tb + 4095) & ~uint64_t(4095);
+}
+
+// ---------------------------------------------------------- layouts
+
+// A validated layout in the working orientation (transposed when the
+// transform transposes, as grid_layout::transpose_or_conjugate does in the
+// reference, grid_layout.hpp:18-25), with one block per cell.
+struct wlayout {
+    std::vector<int32_t> rs, cs;  // working splits
+    int32_t nbr = 0, nbc = 0;     // working block rows / cols
+    std::vector<cell> cells;      // [i * nbc + j]
+    int32_t n_ranks = 0;
+};
+
+int check_splits(const char *what, const int32_t *s, int32_t n) {
+    if (n < 2 || !s) return fail(G2G_ERR_INVALID, "%s: need at least 2 split entries", what);
+    if (s[0] < 0) return fail(G2G_ERR_INVALID, "%s: splits must be non-negative", what);
+    for (int32_t i = 1; i < n; ++i)
+        if (s[i] <= s[i - 1])
+            return fail(G2G_ERR_INVALID, "%s: splits must be strictly increasing (entry %d)", what, i);
+    return G2G_OK;
+}
+
+int load_layout(const char *what, const g2g_layout *l, uint32_t es, bool transpose, wlayout &w) {
+    if (!l) return fail(G2G_ERR_INVALID, "%s layout is NULL", what);
+    G2G_TRY(check_splits(what, l->rows_split, l->n_rows_split));
+    G2G_TRY(check_splits(what, l->cols_split, l->n_cols_split));
+    if (l->n_ranks < 1 || l->n_ranks > 65535)
+        return fail(G2G_ERR_INVALID, "%s: n_ranks must be in [1, 65535]", what);
+    const int32_t br = l->n_rows_split - 1, bc = l->n_cols_split - 1;
+    if (!l->owners) return fail(G2G_ERR_INVALID, "%s: owners is NULL", what);
+    if (l->n_blocks < 0 || (l->n_blocks > 0 && !l->blocks))
+        return fail(G2G_ERR_INVALID, "%s: bad blocks array", what);
+    std::vector<cell> sc((size_t)br * bc);
+    std::vector<uint8_t> seen((size_t)br * bc, 0);
+    for (int64_t k = 0; k < (int64_t)br * bc; ++k) {
+        const int32_t o = l->owners[k];
+        if (o < 0 || o >= l->n_ranks)
+            return fail(G2G_ERR_INVALID, "%s: owner %d of cell %lld outside [0, %d)", what, o, (long long)k,
+                        l->n_ranks);
+        sc[k].owner = o;
+        sc[k].pad = 0;
+    }
+    for (int32_t b = 0; b < l->n_blocks; ++b) {
+        const g2g_block &blk = l->blocks[b];
+        if (blk.row < 0 || blk.row >= br || blk.col < 0 || blk.col >= bc)
+            return fail(G2G_ERR_INVALID,
+                        "%s: block %d coordinates (%d, %d) do not belong to the grid", what, b, blk.row,
+                        blk.col);
+        const int64_t k = (int64_t)blk.row * bc + blk.col;
+        if (seen[k]) return fail(G2G_ERR_INVALID, "%s: cell (%d, %d) has two blocks", what, blk.row, blk.col);
+        seen[k] = 1;
+        const int64_t rows = l->rows_split[blk.row + 1] - l->rows_split[blk.row];
+        if (blk.stride < rows || blk.stride > 0xffffffffll)
+            return fail(G2G_ERR_INVALID, "%s: block (%d, %d) stride %lld invalid for %lld rows", what, blk.row,
+                        blk.col, (long long)blk.stride, (long long)rows);
+        if (!blk.data) return fail(G2G_ERR_INVALID, "%s: block (%d, %d) data is NULL", what, blk.row, blk.col);
+        if (((uintptr_t)blk.data) % (es >= 8 ? 8 : es))
+            return fail(G2G_ERR_INVALID, "%s: block (%d, %d) data not aligned to its element", what, blk.row,
+                        blk.col);
+        sc[k].ptr = (uint64_t)(uintptr_t)blk.data;
+        sc[k].ld = (uint64_t)blk.stride;
+    }
+    for (int64_t k = 0; k < (int64_t)br * bc; ++k)
+        if (!seen[k])
+            return fail(G2G_ERR_INVALID, "%s: cell (%lld, %lld) has no block", what, (long long)(k / bc),
+                        (long long)(k % bc));
+    w.n_ranks = l->n_ranks;
+    if (!transpose) {
+        w.rs.assign(l->rows_split, l->rows_split + l->n_rows_split);
+        w.cs.assign(l->cols_split, l->cols_split + l->n_cols_split);
+        w.nbr = br;
+        w.nbc = bc;
+        w.cells = std::move(sc);
+    } else {
+        // assigned_grid2D::transpose (reference grid2D.cpp:47-50, 93-110)
+        w.rs.assign(l->cols_split, l->cols_split + l->n_cols_split);
+        w.cs.assign(l->rows_split, l->rows_split + l->n_rows_split);
+        w.nbr = bc;
+        w.nbc = br;
+        w.cells.resize(sc.size());
+        for (int32_t i = 0; i < br; ++i)
+            for (int32_t j = 0; j < bc; ++j) w.cells[(size_t)j * br + i] = sc[(size_t)i * bc + j];
+    }
+    return G2G_OK;
+}
+
+// The grid cover along one dimension: the union of the final and initial
+// splits as intervals, each tagged with the final and initial block that
+// contain it. This is the interval table equivalent of the reference's
+// get_decomp_cover (grid_cover.cpp:54-121), which lists, per block of one
+// grid, the range of blocks of the other grid it overlaps.
+int union_spans(const std::vector<int32_t> &f, const std::vector<int32_t> &i, const char *dim,
+                std::vector<span> &out) {
+    if (f.front() != i.front() || f.back() != i.back())
+        return fail(G2G_ERR_INVALID,
+                    "%s of the initial layout (after op) [%d, %d) and the final layout [%d, %d) differ", dim,
+                    i.front(), i.back(), f.front(), f.back());
+    out.clear();
+    size_t a = 0, b = 0;
+    int32_t pos = f.front();
+    while (pos < f.back()) {
+        const int32_t next = std::min(f[a + 1], i[b + 1]);
+        out.push_back(span{pos, next, (int32_t)a, (int32_t)b});
+        pos = next;
+        if (f[a + 1] == pos) ++a;
+        if (i[b + 1] == pos) ++b;
+    }
+    return G2G_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- plans
+
+struct stage_table {
+    dbuf<piece> pieces;
+    dbuf<uint64_t> pend;
+    dbuf<uint32_t> first;
+    int64_t n = 0;
+    uint64_t total = 0;
+    uint64_t tile_bytes = 0;
+    uint32_t n_tiles = 0;
+};
+
+struct g2g_plan_s {
+    int dtype = 0;
+    char trans = 'N';
+    uint32_t es = 0;
+    int32_t P = 0;
+    int64_t m = 0, n = 0;
+    std::vector<span> urows, ucols;  // host copy of the interval table
+
+    stage_table fused;               // all pieces, fused gather
+    dbuf<uint32_t> ranks;            // (src << 16) | dst per piece
+    dbuf<uint32_t> queue;
+
+    // wire order (computed on demand)
+    bool wire_ready = false;
+    dbuf<uint64_t> within;
+    std::vector<uint64_t> counts;    // [s * P + d] elements
+    std::vector<uint64_t> sbase, sdspl, rbase, rdspl;
+    int64_t remote_elems = 0;
+
+    // plain form (allocated on demand)
+    bool staged_ready = false;
+    dbuf<unsigned char> send_arena, recv_arena;
+    stage_table sa, sb, sc;
+};
+
+namespace {
+
+bool persistent_mode(uint32_t n_tiles) {
+    const int p = g_persistent.load();
+    if (p >= 0) return p != 0;
+    return n_tiles > (uint32_t)device_cus() * 64u;
+}
+
+int grid_limit() { return device_cus() * 8; }
+
+// bytes -> inclusive scan -> first piece per tile for a stage table whose
+// pieces are already on the device.
+int finalize_table(stage_table &t, const request *reqtab, const kreqs &kr, uint64_t total_hint,
+                   hipStream_t s) {
+    G2G_TRY(t.pend.alloc((size_t)t.n));
+    G2G_HIP(launch_piece_bytes(t.pieces.p, t.n, reqtab, kr, t.pend.p, s));
+    dbuf<uint64_t> scratch;
+    G2G_TRY(scratch.alloc(scan_scratch_elems(t.n)));
+    G2G_HIP(inclusive_scan_u64(t.pend.p, t.n, scratch.p, s));
+    uint64_t total = 0;
+    if (t.n) G2G_HIP(hipMemcpyAsync(&total, t.pend.p + t.n - 1, sizeof total, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    if (total_hint && total != total_hint)
+        return fail(G2G_ERR_INVALID, "internal: piece bytes %llu != matrix bytes %llu",
+                    (unsigned long long)total, (unsigned long long)total_hint);
+    t.total = total;
+    t.tile_bytes = choose_tile_bytes(total);
+    const uint64_t nt = (total + t.tile_bytes - 1) / t.tile_bytes;
+    if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
+    t.n_tiles = (uint32_t)nt;
+    G2G_TRY(t.first.alloc(nt));
+    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, t.first.p, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+bool is_one(int dtype, const void *a) {
+    switch (dtype) {
+    case G2G_FLOAT32: return *(const float *)a == 1.0f;
+    case G2G_FLOAT64: return *(const double *)a == 1.0;
+    case G2G_COMPLEX64: return ((const float *)a)[0] == 1.0f && ((const float *)a)[1] == 0.0f;
+    default: return ((const double *)a)[0] == 1.0 && ((const double *)a)[1] == 0.0;
+    }
+}
+
+bool is_zero(int dtype, const void *a) {
+    switch (dtype) {
+    case G2G_FLOAT32: return *(const float *)a == 0.0f;
+    case G2G_FLOAT64: return *(const double *)a == 0.0;
+    case G2G_COMPLEX64: return ((const float *)a)[0] == 0.0f && ((const float *)a)[1] == 0.0f;
+    default: return ((const double *)a)[0] == 0.0 && ((const double *)a)[1] == 0.0;
+    }
+}
+
+// Request for a plan: the transform's op, and scaling unless (alpha, beta)
+// is exactly (1, 0), the reference's copy shortcut (memory_utils.hpp:32-35).
+request make_request(int dtype, char trans, const void *alpha, const void *beta, bool with_op,
+                     bool with_scale) {
+    request r;
+    std::memset(&r, 0, sizeof r);
+    r.dtype = (uint32_t)dtype;
+    r.esize = dtype_size(dtype);
+    if (with_op && (trans == 'T' || trans == 'C')) r.flags |= REQ_TRANS;
+    if (with_op && trans == 'C' && dtype_complex(dtype)) r.flags |= REQ_CONJ;
+    const void *a = alpha, *b = beta;
+    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    static const float onef[2] = {1.0f, 0.0f}, zerof[2] = {0.0f, 0.0f};
+    const bool single = dtype == G2G_FLOAT32 || dtype == G2G_COMPLEX64;
+    if (!a) a = single ? (const void *)onef : (const void *)one;
+    if (!b) b = single ? (const void *)zerof : (const void *)zero;
+    if (with_scale && !(is_one(dtype, a) && is_zero(dtype, b))) r.flags |= REQ_SCALE;
+    std::memcpy(r.alpha, a, r.esize);
+    std::memcpy(r.beta, b, r.esize);
+    return r;
+}
+
+int ensure_wire(g2g_plan p, hipStream_t s) {
+    if (p->wire_ready) return G2G_OK;
+    const int32_t P = p->P;
+    if ((int64_t)P * P * 8 > 64 * 1024)
+        return fail(G2G_ERR_UNSUPPORTED, "wire order needs P*P*8 <= 64 KiB of LDS (P = %d)", P);
+    const int64_t n = p->fused.n;
+    const int64_t n_chunks = std::max<int64_t>(1, (n + 2047) / 2048);
+    dbuf<uint64_t> hist, pc;
+    G2G_TRY(hist.alloc((size_t)P * P * n_chunks));
+    G2G_TRY(pc.alloc((size_t)P * P));
+    G2G_TRY(p->within.alloc((size_t)n));
+    G2G_HIP(launch_pair_offsets(p->ranks.p, p->fused.pieces.p, n, P, hist.p, n_chunks, p->within.p, pc.p, s));
+    p->counts.assign((size_t)P * P, 0);
+    G2G_HIP(hipMemcpyAsync(p->counts.data(), pc.p, sizeof(uint64_t) * P * P, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    // communication_data.cpp:87-89: dspls = exclusive prefix of counts,
+    // per rank, over peers; the per-rank buffers are laid out rank after rank.
+    p->sbase.assign(P, 0);
+    p->rbase.assign(P, 0);
+    p->sdspl.assign((size_t)P * P, 0);
+    p->rdspl.assign((size_t)P * P, 0);
+    uint64_t acc = 0;
+    for (int32_t sr = 0; sr < P; ++sr) {
+        p->sbase[sr] = acc;
+        uint64_t d_acc = 0;
+        for (int32_t d = 0; d < P; ++d) {
+            p->sdspl[(size_t)sr * P + d] = d_acc;
+            d_acc += p->counts[(size_t)sr * P + d];
+        }
+        acc += d_acc;
+    }
+    p->remote_elems = (int64_t)acc;
+    acc = 0;
+    for (int32_t d = 0; d < P; ++d) {
+        p->rbase[d] = acc;
+        uint64_t s_acc = 0;
+        for (int32_t sr = 0; sr < P; ++sr) {
+            p->rdspl[(size_t)d * P + sr] = s_acc;
+            s_acc += p->counts[(size_t)sr * P + d];
+        }
+        acc += s_acc;
+    }
+    p->wire_ready = true;
+    return G2G_OK;
+}
+
+int ensure_staged(g2g_plan p, hipStream_t s) {
+    if (p->staged_ready) return G2G_OK;
+    G2G_TRY(ensure_wire(p, s));
+    const int32_t P = p->P;
+    const uint64_t es = p->es;
+    G2G_TRY(p->send_arena.alloc((size_t)p->remote_elems * es));
+    G2G_TRY(p->recv_arena.alloc((size_t)p->remote_elems * es));
+    dbuf<uint64_t> d_sbase, d_sdspl, d_rbase, d_rdspl;
+    G2G_TRY(d_sbase.upload(p->sbase.data(), P, s));
+    G2G_TRY(d_sdspl.upload(p->sdspl.data(), (size_t)P * P, s));
+    G2G_TRY(d_rbase.upload(p->rbase.data(), P, s));
+    G2G_TRY(d_rdspl.upload(p->rdspl.data(), (size_t)P * P, s));
+    const int64_t n = p->fused.n;
+    G2G_TRY(p->sa.pieces.alloc((size_t)n));
+    G2G_TRY(p->sc.pieces.alloc((size_t)n));
+    p->sa.n = p->sc.n = n;
+    G2G_HIP(launch_stage_pieces(p->fused.pieces.p, p->ranks.p, p->within.p, n, P, d_sbase.p, d_sdspl.p,
+                                d_rbase.p, d_rdspl.p, (uint64_t)(uintptr_t)p->send_arena.p,
+                                (uint64_t)(uintptr_t)p->recv_arena.p, p->es, p->sa.pieces.p,
+                                p->sc.pieces.p, s));
+    // Stage B: one contiguous segment per (source, destination) pair with a
+    // nonzero count, as MPI_Isend/MPI_Irecv of transform.cpp:53-87 move them.
+    std::vector<piece> seg;
+    const uint64_t send0 = (uint64_t)(uintptr_t)p->send_arena.p, recv0 = (uint64_t)(uintptr_t)p->recv_arena.p;
+    for (int32_t sr = 0; sr < P; ++sr)
+        for (int32_t d = 0; d < P; ++d) {
+            uint64_t cnt = p->counts[(size_t)sr * P + d];
+            uint64_t so = p->sbase[sr] + p->sdspl[(size_t)sr * P + d];
+            uint64_t ro = p->rbase[d] + p->rdspl[(size_t)d * P + sr];
+            while (cnt) {
+                const uint64_t len = std::min<uint64_t>(cnt, 1u << 30);
+                piece pc;
+                std::memset(&pc, 0, sizeof pc);
+                pc.src = send0 + so * es;
+                pc.dst = recv0 + ro * es;
+                pc.lds = pc.ldd = pc.rows = (uint32_t)len;
+                pc.cols = 1;
+                seg.push_back(pc);
+                cnt -= len;
+                so += len;
+                ro += len;
+            }
+        }
+    p->sb.n = (int64_t)seg.size();
+    G2G_TRY(p->sb.pieces.upload(seg.data(), seg.size(), s));
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, false, false);
+    kr.r[1] = kr.r[0];
+    G2G_TRY(finalize_table(p->sa, nullptr, kr, (uint64_t)p->m * p->n * es, s));
+    G2G_TRY(finalize_table(p->sb, nullptr, kr, (uint64_t)p->remote_elems * es, s));
+    G2G_TRY(finalize_table(p->sc, nullptr, kr, (uint64_t)p->remote_elems * es, s));
+    p->staged_ready = true;
+    return G2G_OK;
+}
+
+int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
+    gather_args a;
+    std::memset(&a, 0, sizeof a);
+    a.pieces = t.pieces.p;
+    a.pend = t.pend.p;
+    a.first_piece = t.first.p;
+    a.reqtab = nullptr;
+    a.kr = kr;
+    a.tile_bytes = t.tile_bytes;
+    a.total_bytes = t.total;
+    a.n_tiles = t.n_tiles;
+    a.queue = p->queue.p;
+    G2G_HIP(launch_gather(a, persistent_mode(t.n_tiles), grid_limit(), s));
+    return G2G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *g2g_last_error(void) { return t_last_error.c_str(); }
+
+int g2g_version(void) { return G2G_ABI_VERSION; }
+
+int g2g_set_tuning(int64_t tile_bytes, int persistent) {
+    if (tile_bytes != 0 && (tile_bytes < 4096 || tile_bytes > (1ll << 26)))
+        return fail(G2G_ERR_INVALID, "tile_bytes %lld outside [4096, 2^26]", (long long)tile_bytes);
+    g_tile_bytes.store(tile_bytes);
+    g_persistent.store(persistent < 0 ? -1 : (persistent ? 1 : 0));
+    return G2G_OK;
+}
+
+int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dtype, char trans,
+                    void *stream, g2g_plan *out) {
+    if (!out) return fail(G2G_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    const uint32_t es = dtype_size(dtype);
+    if (!es) return fail(G2G_ERR_INVALID, "unknown dtype %d", dtype);
+    trans = (char)std::toupper((unsigned char)trans);
+    if (trans != 'N' && trans != 'T' && trans != 'C')
+        return fail(G2G_ERR_INVALID, "trans must be 'N', 'T' or 'C'");
+    hipStream_t s = (hipStream_t)stream;
+    wlayout wi, wf;
+    G2G_TRY(load_layout("initial", initial, es, trans != 'N', wi));
+    G2G_TRY(load_layout("final", final_, es, false, wf));
+    g2g_plan p = new (std::nothrow) g2g_plan_s();
+    if (!p) return fail(G2G_ERR_NOMEM, "out of host memory");
+    struct guard {
+        g2g_plan &p;
+        ~guard() { delete p; }
+    } g{p};
+    p->dtype = dtype;
+    p->trans = trans;
+    p->es = es;
+    p->P = std::max(wi.n_ranks, wf.n_ranks);
+    p->m = wf.rs.back() - wf.rs.front();
+    p->n = wf.cs.back() - wf.cs.front();
+    G2G_TRY(union_spans(wf.rs, wi.rs, "rows", p->urows));
+    G2G_TRY(union_spans(wf.cs, wi.cs, "columns", p->ucols));
+    const int64_t n_ur = (int64_t)p->urows.size(), n_uc = (int64_t)p->ucols.size();
+    const int64_t np = n_ur * n_uc;
+    if (np > 0xffffffffll) return fail(G2G_ERR_UNSUPPORTED, "too many pieces (%lld)", (long long)np);
+    for (const span &r : p->urows)
+        if ((uint64_t)(r.end - r.start) * es > 0xffffffffull)
+            return fail(G2G_ERR_UNSUPPORTED, "piece column exceeds 4 GiB");
+    dbuf<span> d_ur, d_uc;
+    dbuf<int32_t> d_irs, d_ics, d_frs, d_fcs;
+    dbuf<cell> d_ic, d_fc;
+    G2G_TRY(d_ur.upload(p->urows.data(), p->urows.size(), s));
+    G2G_TRY(d_uc.upload(p->ucols.data(), p->ucols.size(), s));
+    G2G_TRY(d_irs.upload(wi.rs.data(), wi.rs.size(), s));
+    G2G_TRY(d_ics.upload(wi.cs.data(), wi.cs.size(), s));
+    G2G_TRY(d_frs.upload(wf.rs.data(), wf.rs.size(), s));
+    G2G_TRY(d_fcs.upload(wf.cs.data(), wf.cs.size(), s));
+    G2G_TRY(d_ic.upload(wi.cells.data(), wi.cells.size(), s));
+    G2G_TRY(d_fc.upload(wf.cells.data(), wf.cells.size(), s));
+    p->fused.n = np;
+    G2G_TRY(p->fused.pieces.alloc((size_t)np));
+    G2G_TRY(p->fused.pend.alloc((size_t)np));
+    G2G_TRY(p->ranks.alloc((size_t)np));
+    G2G_TRY(p->queue.alloc(1));
+    G2G_HIP(launch_plan_pieces(d_ur.p, (int32_t)n_ur, d_uc.p, (int32_t)n_uc, d_irs.p, d_ics.p, d_frs.p, d_fcs.p,
+                               d_ic.p, wi.nbc, d_fc.p, wf.nbc, es, trans != 'N', p->fused.pieces.p,
+                               p->fused.pend.p, p->ranks.p, s));
+    dbuf<uint64_t> scratch;
+    G2G_TRY(scratch.alloc(scan_scratch_elems(np)));
+    G2G_HIP(inclusive_scan_u64(p->fused.pend.p, np, scratch.p, s));
+    const uint64_t total = (uint64_t)p->m * p->n * es;
+    p->fused.total = total;
+    p->fused.tile_bytes = choose_tile_bytes(total);
+    const uint64_t nt = (total + p->fused.tile_bytes - 1) / p->fused.tile_bytes;
+    if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
+    p->fused.n_tiles = (uint32_t)nt;
+    G2G_TRY(p->fused.first.alloc(nt));
+    G2G_HIP(launch_first_piece(p->fused.pend.p, np, p->fused.tile_bytes, p->fused.first.p, s));
+    uint64_t check = 0;
+    if (np) G2G_HIP(hipMemcpyAsync(&check, p->fused.pend.p + np - 1, sizeof check, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    if (check != total)
+        return fail(G2G_ERR_INVALID, "internal: pieces cover %llu bytes, matrix has %llu",
+                    (unsigned long long)check, (unsigned long long)total);
+    *out = p;
+    p = nullptr;  // released from the guard
+    return G2G_OK;
+}
+
+int g2g_plan_destroy(g2g_plan plan) {
+    delete plan;
+    return G2G_OK;
+}
+
+int g2g_plan_get_info(g2g_plan p, g2g_plan_info *info) {
+    if (!p || !info) return fail(G2G_ERR_INVALID, "NULL argument");
+    std::memset(info, 0, sizeof *info);
+    info->m = p->m;
+    info->n = p->n;
+    info->n_pieces = p->fused.n;
+    info->bytes = (int64_t)p->fused.total;
+    info->remote_bytes = p->wire_ready ? p->remote_elems * (int64_t)p->es : -1;
+    info->n_tiles = p->fused.n_tiles;
+    info->tile_bytes = (int64_t)p->fused.tile_bytes;
+    info->n_ranks = p->P;
+    info->dtype = p->dtype;
+    info->trans = p->trans;
+    return G2G_OK;
+}
+
+int g2g_transform(g2g_plan p, const void *alpha, const void *beta, void *stream) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    if ((alpha == nullptr) != (beta == nullptr)) return fail(G2G_ERR_INVALID, "give both alpha and beta or neither");
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    kr.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, true);
+    kr.r[1] = kr.r[0];
+    return run_table(p, p->fused, kr, (hipStream_t)stream);
+}
+
+int g2g_transform_staged(g2g_plan p, const void *alpha, const void *beta, void *stream) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    if ((alpha == nullptr) != (beta == nullptr)) return fail(G2G_ERR_INVALID, "give both alpha and beta or neither");
+    hipStream_t s = (hipStream_t)stream;
+    G2G_TRY(ensure_staged(p, s));
+    kreqs ka, kb, kc;
+    std::memset(&ka, 0, sizeof ka);
+    ka.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, false);  // pack
+    ka.r[1] = make_request(p->dtype, p->trans, alpha, beta, true, true);   // local copies
+    kb = ka;
+    kb.r[0] = make_request(p->dtype, 'N', nullptr, nullptr, false, false);  // exchange
+    kb.r[1] = kb.r[0];
+    kc = ka;
+    kc.r[0] = make_request(p->dtype, 'N', alpha, beta, false, true);        // unpack
+    kc.r[1] = kc.r[0];
+    G2G_TRY(run_table(p, p->sa, ka, s));
+    G2G_TRY(run_table(p, p->sb, kb, s));
+    G2G_TRY(run_table(p, p->sc, kc, s));
+    return G2G_OK;
+}
+
+int g2g_plan_counts(g2g_plan p, int64_t *counts, int64_t capacity) {
+    if (!p || !counts) return fail(G2G_ERR_INVALID, "NULL argument");
+    if (capacity < (int64_t)p->P * p->P) return fail(G2G_ERR_INVALID, "capacity below P*P");
+    G2G_TRY(ensure_wire(p, nullptr));
+    for (size_t k = 0; k < p->counts.size(); ++k) counts[k] = (int64_t)p->counts[k];
+    return G2G_OK;
+}
+
+int g2g_plan_wire(g2g_plan p, int32_t *row0, int32_t *col0, int32_t *rows, int32_t *cols,
+                  int32_t *src_rank, int32_t *dst_rank, int64_t *send_off, int64_t *recv_off,
+                  int64_t capacity) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    const int64_t n = p->fused.n;
+    if (capacity < n) return fail(G2G_ERR_INVALID, "capacity below n_pieces");
+    G2G_TRY(ensure_wire(p, nullptr));
+    std::vector<uint32_t> rk((size_t)n);
+    std::vector<uint64_t> w((size_t)n);
+    if (n) {
+        G2G_HIP(hipMemcpy(rk.data(), p->ranks.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        G2G_HIP(hipMemcpy(w.data(), p->within.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    }
+    const int64_t n_ur = (int64_t)p->urows.size();
+    const int32_t P = p->P;
+    for (int64_t q = 0; q < n; ++q) {
+        const span &R = p->urows[q % n_ur];
+        const span &C = p->ucols[q / n_ur];
+        const int32_t sr = (int32_t)(rk[q] >> 16), d = (int32_t)(rk[q] & 0xffffu);
+        if (row0) row0[q] = R.start;
+        if (col0) col0[q] = C.start;
+        if (rows) rows[q] = R.end - R.start;
+        if (cols) cols[q] = C.end - C.start;
+        if (src_rank) src_rank[q] = sr;
+        if (dst_rank) dst_rank[q] = d;
+        const bool remote = sr != d;
+        if (send_off) send_off[q] = remote ? (int64_t)(p->sdspl[(size_t)sr * P + d] + w[q]) : -1;
+        if (recv_off) recv_off[q] = remote ? (int64_t)(p->rdspl[(size_t)d * P + sr] + w[q]) : -1;
+    }
+    return G2G_OK;
+}
+
+int g2g_plan_send_buffer(g2g_plan p, int rank, const void **data, int64_t *elems) {
+    if (!p || !data || !elems) return fail(G2G_ERR_INVALID, "NULL argument");
+    if (!p->staged_ready) return fail(G2G_ERR_STATE, "the plain form has not run on this plan");
+    if (rank < 0 || rank >= p->P) return fail(G2G_ERR_INVALID, "rank %d outside [0, %d)", rank, p->P);
+    uint64_t tot = 0;
+    for (int32_t d = 0; d < p->P; ++d) tot += p->counts[(size_t)rank * p->P + d];
+    *data = p->send_arena.p + p->sbase[rank] * p->es;
+    *elems = (int64_t)tot;
+    return G2G_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+}  // extern "C"
+
+struct g2g_batch_s {
+    std::vector<g2g_plan> plans;
+    stage_table t;
+    dbuf<request> reqs;
+    dbuf<uint32_t> queue;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~g2g_batch_s() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+int batch_launch(g2g_batch b, hipStream_t s) {
+    gather_args a;
+    std::memset(&a, 0, sizeof a);
+    a.pieces = b->t.pieces.p;
+    a.pend = b->t.pend.p;
+    a.first_piece = b->t.first.p;
+    a.reqtab = b->reqs.p;
+    a.tile_bytes = b->t.tile_bytes;
+    a.total_bytes = b->t.total;
+    a.n_tiles = b->t.n_tiles;
+    a.queue = b->queue.p;
+    G2G_HIP(launch_gather(a, true, grid_limit(), s));
+    return G2G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
+                     const void *const *betas, void *stream, g2g_batch *out) {
+    if (!out || (n > 0 && !plans) || n < 0) return fail(G2G_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    g2g_batch b = new (std::nothrow) g2g_batch_s();
+    if (!b) return fail(G2G_ERR_NOMEM, "out of host memory");
+    struct guard {
+        g2g_batch &b;
+        ~guard() { delete b; }
+    } g{b};
+    std::vector<request> rq((size_t)std::max(n, 1));
+    int64_t np = 0;
+    for (int k = 0; k < n; ++k) {
+        if (!plans[k]) return fail(G2G_ERR_INVALID, "plan %d is NULL", k);
+        const void *a = alphas ? alphas[k] : nullptr, *be = betas ? betas[k] : nullptr;
+        if ((a == nullptr) != (be == nullptr)) return fail(G2G_ERR_INVALID, "plan %d: alpha/beta mismatch", k);
+        rq[k] = make_request(plans[k]->dtype, plans[k]->trans, a, be, true, true);
+        np += plans[k]->fused.n;
+        b->plans.push_back(plans[k]);
+    }
+    if (np > 0xffffffffll) return fail(G2G_ERR_UNSUPPORTED, "batch has too many pieces");
+    b->t.n = np;
+    G2G_TRY(b->t.pieces.alloc((size_t)np));
+    G2G_TRY(b->reqs.upload(rq.data(), rq.size(), s));
+    G2G_TRY(b->queue.alloc(1));
+    int64_t off = 0;
+    for (int k = 0; k < n; ++k) {
+        G2G_HIP(launch_concat_pieces(plans[k]->fused.pieces.p, plans[k]->fused.n, (uint32_t)k, b->t.pieces.p + off, s));
+        off += plans[k]->fused.n;
+    }
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    G2G_TRY(finalize_table(b->t, b->reqs.p, kr, 0, s));
+    *out = b;
+    b = nullptr;
+    return G2G_OK;
+}
+
+int g2g_batch_execute(g2g_batch b, void *stream) {
+    if (!b) return fail(G2G_ERR_INVALID, "batch is NULL");
+    return batch_launch(b, (hipStream_t)stream);
+}
+
+int g2g_batch_execute_graph(g2g_batch b, void *stream) {
+    if (!b) return fail(G2G_ERR_INVALID, "batch is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(G2G_ERR_INVALID, "graph replay needs a non-null stream");
+    if (!b->exec) {
+        G2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = batch_launch(b, s);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc != G2G_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(G2G_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        b->graph = graph;
+        G2G_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+    }
+    G2G_HIP(hipGraphLaunch(b->exec, s));
+    return G2G_OK;
+}
+
+int g2g_batch_bytes(g2g_batch b, int64_t *bytes) {
+    if (!b || !bytes) return fail(G2G_ERR_INVALID, "NULL argument");
+    *bytes = (int64_t)b->t.total;
+    return G2G_OK;
+}
+
+int g2g_batch_destroy(g2g_batch b) {
+    delete b;
+    return G2G_OK;
+}
+
+// ------------------------------------------------------ test support
+
+static int fill_table(const g2g_layout *l, std::vector<fill_block> &fb, int64_t &n_cols) {
+    if (!l || !l->rows_split || !l->cols_split || l->n_rows_split < 2 || l->n_cols_split < 2)
+        return fail(G2G_ERR_INVALID, "bad layout");
+    const int32_t br = l->n_rows_split - 1, bc = l->n_cols_split - 1;
+    fb.clear();
+    for (int32_t b = 0; b < l->n_blocks; ++b) {
+        const g2g_block &x = l->blocks[b];
+        if (x.row < 0 || x.row >= br || x.col < 0 || x.col >= bc || !x.data)
+            return fail(G2G_ERR_INVALID, "bad block %d", b);
+        fill_block f;
+        f.ptr = (uint64_t)(uintptr_t)x.data;
+        f.ld = (uint64_t)x.stride;
+        f.r0 = l->rows_split[x.row];
+        f.r1 = l->rows_split[x.row + 1];
+        f.c0 = l->cols_split[x.col];
+        f.c1 = l->cols_split[x.col + 1];
+        fb.push_back(f);
+    }
+    n_cols = l->cols_split[bc];
+    return G2G_OK;
+}
+
+int g2g_fill_layout(const g2g_layout *layout, int dtype, uint64_t seed, void *stream) {
+    if (!dtype_size(dtype)) return fail(G2G_ERR_INVALID, "unknown dtype");
+    std::vector<fill_block> fb;
+    int64_t n_cols = 0;
+    G2G_TRY(fill_table(layout, fb, n_cols));
+    hipStream_t s = (hipStream_t)stream;
+    dbuf<fill_block> d;
+    G2G_TRY(d.upload(fb.data(), fb.size(), s));
+    G2G_HIP(launch_fill(d.p, (int32_t)fb.size(), dtype, seed, n_cols, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+int g2g_verify_layout(const g2g_layout *final_, int dtype, uint64_t seed, char trans,
+                      int64_t a_n_cols, void *stream, int64_t *mismatches) {
+    if (!dtype_size(dtype) || !mismatches) return fail(G2G_ERR_INVALID, "bad arguments");
+    std::vector<fill_block> fb;
+    int64_t n_cols = 0;
+    G2G_TRY(fill_table(final_, fb, n_cols));
+    hipStream_t s = (hipStream_t)stream;
+    dbuf<fill_block> d;
+    dbuf<unsigned long long> bad;
+    G2G_TRY(d.upload(fb.data(), fb.size(), s));
+    G2G_TRY(bad.alloc(1));
+    G2G_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
+    const int t = (trans == 'T' || trans == 'C' || trans == 't' || trans == 'c');
+    const int c = (trans == 'C' || trans == 'c') && dtype_complex(dtype);
+    G2G_HIP(launch_verify(d.p, (int32_t)fb.size(), dtype, seed, t, c, a_n_cols, bad.p, s));
+    unsigned long long h = 0;
+    G2G_HIP(hipMemcpyAsync(&h, bad.p, sizeof h, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    *mismatches = (int64_t)h;
+    return G2G_OK;
+}
+
+}  // extern "C"

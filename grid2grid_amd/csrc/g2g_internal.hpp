@@ -1,0 +1,140 @@
+// Internal declarations shared by the host library and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace g2g {
+
+constexpr int kThreads = 256;    // threads per workgroup of every data-movement kernel
+constexpr int kTileDim = 32;     // transpose tile edge (elements)
+
+enum : uint32_t {
+    REQ_TRANS = 1u,  // destination(i, j) = source(j, i)
+    REQ_CONJ = 2u,   // flip the sign bit of the imaginary part
+    REQ_SCALE = 4u,  // dst = beta*dst + alpha*op(src)
+};
+
+// Per-request parameters: element type, operation and scalars (raw bits).
+struct request {
+    uint32_t dtype;
+    uint32_t flags;
+    uint32_t esize;
+    uint32_t pad;
+    uint64_t alpha[2];
+    uint64_t beta[2];
+};
+static_assert(sizeof(request) == 48, "request layout");
+
+// One rectangle moved by the gather: the intersection of an initial and a
+// final cell ("piece"), or a staging segment. Final orientation: rows x cols.
+// Source element (i, j) lives at src + (j*lds + i)*esize, or at
+// src + (i*lds + j)*esize when the request transposes; destination element
+// (i, j) at dst + (j*ldd + i)*esize.
+struct piece {
+    uint64_t src;
+    uint64_t dst;
+    uint32_t lds;
+    uint32_t ldd;
+    uint32_t rows;
+    uint32_t cols;
+    uint32_t req;
+    uint32_t pad;
+};
+static_assert(sizeof(piece) == 40, "piece layout");
+
+// Device block table entry (one per grid cell).
+struct cell {
+    uint64_t ptr;   // byte address of the block's element (0, 0)
+    uint64_t ld;    // leading dimension in elements
+    int32_t owner;  // virtual rank
+    int32_t pad;
+};
+static_assert(sizeof(cell) == 24, "cell layout");
+
+// Interval of the union of two split vectors: [start, end) lies inside
+// final block `fblk` and initial block `iblk` (indices along this dimension).
+struct span {
+    int32_t start;
+    int32_t end;
+    int32_t fblk;
+    int32_t iblk;
+};
+
+// Request arguments passed by value; used when no device request table is given.
+struct kreqs {
+    request r[2];
+};
+
+// ------------------------------------------------------------- launchers
+
+// pieces[p] for every union cell p = uj * n_ur + ui, plus bytes[p], ranks.
+hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols, int32_t n_uc,
+                              const int32_t *irs, const int32_t *ics, const int32_t *frs,
+                              const int32_t *fcs, const cell *iblk, int32_t n_ic,
+                              const cell *fblk, int32_t n_fc, uint32_t esize, int trans,
+                              piece *pieces, uint64_t *bytes, uint32_t *ranks,
+                              hipStream_t stream);
+
+// bytes[p] = rows*cols*esize of the piece's request (request table or kreqs).
+hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const request *reqtab,
+                              const kreqs &kr, uint64_t *bytes, hipStream_t stream);
+
+// In-place inclusive scan of n uint64 values (scratch: see scan_scratch_elems).
+size_t scan_scratch_elems(int64_t n);
+hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipStream_t stream);
+
+// first_piece[t] = the piece containing byte t*tile_bytes of the piece stream.
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes,
+                              uint32_t *first_piece, hipStream_t stream);
+
+// Wire order of the plain form: per piece, the element offset inside its
+// (source, destination) pair's segment, pieces ordered by position p
+// (= the reference's (cols interval, rows interval) order); totals per pair.
+hipError_t launch_pair_offsets(const uint32_t *ranks, const piece *pieces, int64_t n,
+                               int32_t n_ranks, uint64_t *hist, int64_t n_chunks,
+                               uint64_t *within, uint64_t *pair_counts, hipStream_t stream);
+
+// Stage tables of the plain form from pieces, ranks, within-pair offsets and
+// per-rank bases/displacements (elements).
+hipError_t launch_stage_pieces(const piece *pieces, const uint32_t *ranks, const uint64_t *within,
+                               int64_t n, int32_t n_ranks, const uint64_t *sbase,
+                               const uint64_t *sdspl, const uint64_t *rbase,
+                               const uint64_t *rdspl, uint64_t send_arena, uint64_t recv_arena,
+                               uint32_t esize, piece *stage_a, piece *stage_c,
+                               hipStream_t stream);
+
+// Copies n pieces setting their request index.
+hipError_t launch_concat_pieces(const piece *src, int64_t n, uint32_t req, piece *dst,
+                                hipStream_t stream);
+
+// The gather over a finalized piece table.
+struct gather_args {
+    const piece *pieces;
+    const uint64_t *pend;
+    const uint32_t *first_piece;
+    const request *reqtab;  // may be null: kr is used
+    kreqs kr;
+    uint64_t tile_bytes;
+    uint64_t total_bytes;
+    uint32_t n_tiles;
+    uint32_t *queue;        // persistent mode work counter (reset by the launcher)
+};
+hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
+                         hipStream_t stream);
+
+// Test-support kernels.
+struct fill_block {
+    uint64_t ptr;
+    uint64_t ld;
+    int32_t r0, r1, c0, c1;
+};
+hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
+                       int64_t n_cols, hipStream_t stream);
+hipError_t launch_verify(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
+                         int trans, int conj, int64_t a_n_cols, unsigned long long *bad,
+                         hipStream_t stream);
+
+}  // namespace g2g

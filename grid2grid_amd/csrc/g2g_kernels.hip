@@ -1,0 +1,757 @@
+// HIP kernels of the MI355X grid2grid transform (gfx950, wave64).
+//
+// Data movement only: elements move as bit patterns through 4/8/16-byte
+// vector registers; the one arithmetic path (beta*dst + alpha*op(src), used
+// only when the caller scales) is compiled without FMA contraction.
+#include "g2g_internal.hpp"
+
+#include <algorithm>
+
+namespace g2g {
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int B> struct bits_t;
+template <> struct bits_t<4> { using type = uint32_t; };
+template <> struct bits_t<8> { using type = u32x2; };
+template <> struct bits_t<16> { using type = u32x4; };
+
+enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
+
+// ------------------------------------------------------------ element ops
+
+__device__ __forceinline__ double u2d(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ u32x2 d2u(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    return u32x2{(uint32_t)b, (uint32_t)(b >> 32)};
+}
+
+// beta*dst + alpha*src, evaluated as round(round(beta*dst) + round(alpha*src))
+// (memory_utils.hpp:37 in the reference, whose x86-64 build has no FMA);
+// complex products are (ac - bd, ad + bc).
+#pragma clang fp contract(off)
+__device__ __forceinline__ uint32_t scale4(uint32_t s, uint32_t d, const request &rq) {
+    const float a = __uint_as_float((uint32_t)rq.alpha[0]);
+    const float b = __uint_as_float((uint32_t)rq.beta[0]);
+    const float bd = b * __uint_as_float(d);
+    const float as = a * __uint_as_float(s);
+    return __float_as_uint(bd + as);
+}
+
+__device__ __forceinline__ u32x2 scale8(u32x2 s, u32x2 d, const request &rq) {
+    u32x2 r;
+    if (rq.dtype == DT_F64) {
+        const double a = __longlong_as_double((long long)rq.alpha[0]);
+        const double b = __longlong_as_double((long long)rq.beta[0]);
+        const double bd = b * u2d(d.x, d.y);
+        const double as = a * u2d(s.x, s.y);
+        r = d2u(bd + as);
+    } else {  // complex<float>
+        const float ar = __uint_as_float((uint32_t)rq.alpha[0]);
+        const float ai = __uint_as_float((uint32_t)(rq.alpha[0] >> 32));
+        const float br = __uint_as_float((uint32_t)rq.beta[0]);
+        const float bi = __uint_as_float((uint32_t)(rq.beta[0] >> 32));
+        const float dr = __uint_as_float(d.x), di = __uint_as_float(d.y);
+        const float sr = __uint_as_float(s.x), si = __uint_as_float(s.y);
+        const float bdr0 = br * dr, bdr1 = bi * di;
+        const float bdi0 = br * di, bdi1 = bi * dr;
+        const float asr0 = ar * sr, asr1 = ai * si;
+        const float asi0 = ar * si, asi1 = ai * sr;
+        const float bd_re = bdr0 - bdr1, bd_im = bdi0 + bdi1;
+        const float as_re = asr0 - asr1, as_im = asi0 + asi1;
+        r.x = __float_as_uint(bd_re + as_re);
+        r.y = __float_as_uint(bd_im + as_im);
+    }
+    return r;
+}
+
+__device__ __forceinline__ u32x4 scale16(u32x4 s, u32x4 d, const request &rq) {
+    const double ar = __longlong_as_double((long long)rq.alpha[0]);
+    const double ai = __longlong_as_double((long long)rq.alpha[1]);
+    const double br = __longlong_as_double((long long)rq.beta[0]);
+    const double bi = __longlong_as_double((long long)rq.beta[1]);
+    const double dr = u2d(d.x, d.y), di = u2d(d.z, d.w);
+    const double sr = u2d(s.x, s.y), si = u2d(s.z, s.w);
+    const double bdr0 = br * dr, bdr1 = bi * di;
+    const double bdi0 = br * di, bdi1 = bi * dr;
+    const double asr0 = ar * sr, asr1 = ai * si;
+    const double asi0 = ar * si, asi1 = ai * sr;
+    const double bd_re = bdr0 - bdr1, bd_im = bdi0 + bdi1;
+    const double as_re = asr0 - asr1, as_im = asi0 + asi1;
+    const u32x2 re = d2u(bd_re + as_re), im = d2u(bd_im + as_im);
+    return u32x4{re.x, re.y, im.x, im.y};
+}
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ uint32_t scale_bits(uint32_t s, uint32_t d, const request &rq) {
+    return scale4(s, d, rq);
+}
+__device__ __forceinline__ u32x2 scale_bits(u32x2 s, u32x2 d, const request &rq) {
+    return scale8(s, d, rq);
+}
+__device__ __forceinline__ u32x4 scale_bits(u32x4 s, u32x4 d, const request &rq) {
+    return scale16(s, d, rq);
+}
+
+// std::conj: sign bit of the imaginary part (the host sets REQ_CONJ only for
+// complex types; conjugate() of real types is the identity).
+__device__ __forceinline__ uint32_t conj_bits(uint32_t v) { return v; }
+__device__ __forceinline__ u32x2 conj_bits(u32x2 v) {
+    v.y ^= 0x80000000u;
+    return v;
+}
+__device__ __forceinline__ u32x4 conj_bits(u32x4 v) {
+    v.w ^= 0x80000000u;
+    return v;
+}
+
+// ---------------------------------------------------------- copy paths
+
+__device__ __forceinline__ uint32_t ceil_log2(uint32_t v) {
+    return v <= 1 ? 0u : 32u - (uint32_t)__clz(v - 1);
+}
+
+// Column-contiguous copy of ncols columns of vpc V-byte vectors each.
+// Lanes are grouped TX per column (TX = next power of two >= vpc, <= 64), so a
+// wave covers 64/TX columns of a short piece or 64 vectors of a long column.
+// Each thread keeps U = 64/V vectors (64 bytes) in flight before storing.
+template <int V>
+__device__ __forceinline__ void copy_cols(const unsigned char *__restrict__ s,
+                                          unsigned char *__restrict__ d, uint64_t lds_b,
+                                          uint64_t ldd_b, uint32_t vpc, uint32_t ncols) {
+    using T = typename bits_t<V>::type;
+    constexpr int U = 64 / V;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t txl = vpc >= 64 ? 6u : ceil_log2(vpc);
+    const uint32_t TX = 1u << txl;
+    const uint32_t NY = (uint32_t)kThreads >> txl;
+    const uint32_t tx = tid & (TX - 1);
+    const uint32_t ty = tid >> txl;
+    for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
+        for (uint32_t v = tx; v < vpc; v += TX) {
+            T r[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t c = cb + (uint32_t)u * NY;
+                if (c < ncols) r[u] = *(const T *)(s + c * lds_b + (uint64_t)v * V);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t c = cb + (uint32_t)u * NY;
+                if (c < ncols) *(T *)(d + c * ldd_b + (uint64_t)v * V) = r[u];
+            }
+        }
+    }
+}
+
+// Scaled column copy, one element per lane slot.
+template <int ES>
+__device__ __forceinline__ void copy_cols_scaled(const unsigned char *__restrict__ s,
+                                                 unsigned char *__restrict__ d, uint64_t lds_b,
+                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
+                                                 const request &rq) {
+    using T = typename bits_t<ES>::type;
+    constexpr int U = 4;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t txl = rows >= 64 ? 6u : ceil_log2(rows);
+    const uint32_t TX = 1u << txl;
+    const uint32_t NY = (uint32_t)kThreads >> txl;
+    const uint32_t tx = tid & (TX - 1);
+    const uint32_t ty = tid >> txl;
+    for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
+        for (uint32_t v = tx; v < rows; v += TX) {
+            T a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t c = cb + (uint32_t)u * NY;
+                if (c < ncols) {
+                    a[u] = *(const T *)(s + c * lds_b + (uint64_t)v * ES);
+                    b[u] = *(const T *)(d + c * ldd_b + (uint64_t)v * ES);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t c = cb + (uint32_t)u * NY;
+                if (c < ncols) *(T *)(d + c * ldd_b + (uint64_t)v * ES) = scale_bits(a[u], b[u], rq);
+            }
+        }
+    }
+}
+
+// Transposing copy through an LDS tile: destination(i, j) = op(source(j, i)).
+// Source element (i, j) (final orientation) is at s + (i*lds + j)*ES, i.e.
+// storage rows run along final columns and are read contiguously; the
+// destination is written contiguously along final rows.
+template <int ES>
+__device__ __forceinline__ void copy_transposed(const unsigned char *__restrict__ s,
+                                                unsigned char *__restrict__ d, uint64_t lds_b,
+                                                uint64_t ldd_b, uint32_t rows, uint32_t ncols,
+                                                const request &rq, unsigned char *smem) {
+    using T = typename bits_t<ES>::type;
+    T(*tile)[kTileDim + 1] = reinterpret_cast<T(*)[kTileDim + 1]>(smem);
+    const uint32_t x = threadIdx.x & (kTileDim - 1);
+    const uint32_t y = threadIdx.x / kTileDim;  // 0..7
+    constexpr uint32_t YS = kThreads / kTileDim;
+    const bool conj = (rq.flags & REQ_CONJ) != 0;
+    const bool scale = (rq.flags & REQ_SCALE) != 0;
+    for (uint32_t jc = 0; jc < ncols; jc += kTileDim) {
+        for (uint32_t ic = 0; ic < rows; ic += kTileDim) {
+#pragma unroll
+            for (uint32_t k = 0; k < kTileDim; k += YS) {
+                const uint32_t i = ic + y + k, j = jc + x;
+                if (i < rows && j < ncols) tile[y + k][x] = *(const T *)(s + i * lds_b + (uint64_t)j * ES);
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < kTileDim; k += YS) {
+                const uint32_t i = ic + x, j = jc + y + k;
+                if (i < rows && j < ncols) {
+                    T v = tile[x][y + k];
+                    if (conj) v = conj_bits(v);
+                    T *dp = (T *)(d + j * ldd_b + (uint64_t)i * ES);
+                    if (scale) v = scale_bits(v, *dp, rq);
+                    *dp = v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void process_piece(const piece &pc, const request &rq, uint32_t c0,
+                                              uint32_t c1, unsigned char *smem) {
+    const uint32_t es = rq.esize;
+    const uint32_t ncols = c1 - c0;
+    const uint64_t lds_b = (uint64_t)pc.lds * es;
+    const uint64_t ldd_b = (uint64_t)pc.ldd * es;
+    unsigned char *d = (unsigned char *)pc.dst + (uint64_t)c0 * ldd_b;
+    if (rq.flags & REQ_TRANS) {
+        const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * es;
+        if (es == 4)
+            copy_transposed<4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+        else if (es == 8)
+            copy_transposed<8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+        else
+            copy_transposed<16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+        return;
+    }
+    const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * lds_b;
+    if (rq.flags & REQ_SCALE) {
+        if (es == 4)
+            copy_cols_scaled<4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+        else if (es == 8)
+            copy_cols_scaled<8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+        else
+            copy_cols_scaled<16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+        return;
+    }
+    const uint32_t colb = pc.rows * es;
+    const uint64_t al = (uint64_t)s | (uint64_t)d | colb | (ncols > 1 ? (lds_b | ldd_b) : 0);
+    if ((al & 15) == 0)
+        copy_cols<16>(s, d, lds_b, ldd_b, colb >> 4, ncols);
+    else if ((al & 7) == 0)
+        copy_cols<8>(s, d, lds_b, ldd_b, colb >> 3, ncols);
+    else
+        copy_cols<4>(s, d, lds_b, ldd_b, colb >> 2, ncols);
+}
+
+// Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
+// stream (pieces concatenated in table order, each column-major). It moves
+// every column (every 32-column group for transposed pieces) whose first
+// byte lies in that range, so work items partition the columns exactly.
+__device__ __forceinline__ void process_tile(const gather_args &a, uint32_t t,
+                                             unsigned char *smem) {
+    const uint64_t lo = (uint64_t)t * a.tile_bytes;
+    const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
+    for (uint32_t p = a.first_piece[t];; ++p) {
+        const uint64_t pstart = p ? a.pend[p - 1] : 0;
+        if (pstart >= hi) break;
+        const piece pc = a.pieces[p];
+        const request rq = a.reqtab ? a.reqtab[pc.req] : (pc.req ? a.kr.r[1] : a.kr.r[0]);
+        const uint64_t colb = (uint64_t)pc.rows * rq.esize;
+        const uint32_t q = (rq.flags & REQ_TRANS) ? (uint32_t)kTileDim : 1u;
+        const uint64_t qb = colb * q;
+        uint64_t c0 = 0;
+        if (lo > pstart) c0 = ((lo - pstart + qb - 1) / qb) * q;
+        uint64_t c1 = ((hi - pstart + qb - 1) / qb) * q;
+        if (c1 > pc.cols) c1 = pc.cols;
+        if (c0 < c1) process_piece(pc, rq, (uint32_t)c0, (uint32_t)c1, smem);
+    }
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kTileDim * (kTileDim + 1) * 16];
+    if (!PERSIST) {
+        process_tile(a, blockIdx.x, smem);
+        return;
+    }
+    __shared__ uint32_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
+        __syncthreads();
+        if (t >= a.n_tiles) break;
+        process_tile(a, t, smem);
+    }
+}
+
+__global__ void k_reset_queue(uint32_t *q) { *q = 0; }
+
+hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
+                         hipStream_t stream) {
+    if (a.n_tiles == 0) return hipSuccess;
+    if (!persistent) {
+        hipLaunchKernelGGL(k_gather<false>, dim3(a.n_tiles), dim3(kThreads), 0, stream, a);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
+    const uint32_t grid = std::min<uint32_t>(a.n_tiles, (uint32_t)std::max(1, grid_limit));
+    hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(kThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------- planning
+
+// One thread per union cell p = uj*n_ur + ui: the piece where initial cell
+// (urows[ui].iblk, ucols[uj].iblk) meets final cell (fblk, fblk). The source
+// pointer follows block::subblock (reference block.cpp:165-196): for a
+// transposed initial layout the storage rows run along final columns.
+__global__ void k_plan_pieces(const span *urows, int32_t n_ur, const span *ucols, int32_t n_uc,
+                              const int32_t *irs, const int32_t *ics, const int32_t *frs,
+                              const int32_t *fcs, const cell *iblk, int32_t n_ic,
+                              const cell *fblk, int32_t n_fc, uint32_t es, int trans,
+                              piece *pieces, uint64_t *bytes, uint32_t *ranks) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)n_ur * n_uc) return;
+    const int32_t ui = (int32_t)(p % n_ur), uj = (int32_t)(p / n_ur);
+    const span R = urows[ui], C = ucols[uj];
+    const cell F = fblk[(int64_t)R.fblk * n_fc + C.fblk];
+    const cell I = iblk[(int64_t)R.iblk * n_ic + C.iblk];
+    const uint32_t rows = (uint32_t)(R.end - R.start), cols = (uint32_t)(C.end - C.start);
+    const uint64_t dst =
+        F.ptr + ((uint64_t)(C.start - fcs[C.fblk]) * F.ld + (uint64_t)(R.start - frs[R.fblk])) * es;
+    uint64_t src;
+    if (!trans)
+        src = I.ptr + ((uint64_t)(C.start - ics[C.iblk]) * I.ld + (uint64_t)(R.start - irs[R.iblk])) * es;
+    else
+        src = I.ptr + ((uint64_t)(R.start - irs[R.iblk]) * I.ld + (uint64_t)(C.start - ics[C.iblk])) * es;
+    piece pc;
+    pc.src = src;
+    pc.dst = dst;
+    pc.lds = (uint32_t)I.ld;
+    pc.ldd = (uint32_t)F.ld;
+    pc.rows = rows;
+    pc.cols = cols;
+    pc.req = 0;
+    pc.pad = 0;
+    pieces[p] = pc;
+    bytes[p] = (uint64_t)rows * cols * es;
+    ranks[p] = ((uint32_t)I.owner << 16) | (uint32_t)F.owner;
+}
+
+hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols, int32_t n_uc,
+                              const int32_t *irs, const int32_t *ics, const int32_t *frs,
+                              const int32_t *fcs, const cell *iblk, int32_t n_ic,
+                              const cell *fblk, int32_t n_fc, uint32_t esize, int trans,
+                              piece *pieces, uint64_t *bytes, uint32_t *ranks,
+                              hipStream_t stream) {
+    const int64_t n = (int64_t)n_ur * n_uc;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_pieces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, urows,
+                       n_ur, ucols, n_uc, irs, ics, frs, fcs, iblk, n_ic, fblk, n_fc, esize, trans,
+                       pieces, bytes, ranks);
+    return hipGetLastError();
+}
+
+__global__ void k_piece_bytes(const piece *pieces, int64_t n, const request *reqtab, kreqs kr,
+                              uint64_t *bytes) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const piece pc = pieces[p];
+    const uint32_t es = reqtab ? reqtab[pc.req].esize : (pc.req ? kr.r[1].esize : kr.r[0].esize);
+    bytes[p] = (uint64_t)pc.rows * pc.cols * es;
+}
+
+hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const request *reqtab,
+                              const kreqs &kr, uint64_t *bytes, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_piece_bytes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       pieces, n, reqtab, kr, bytes);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- prefix scan
+
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = kThreads * kScanItems;  // 2048 values per workgroup
+
+__device__ __forceinline__ uint64_t wave_inclusive_scan(uint64_t x, uint32_t lane) {
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Inclusive scan of one 2048-value block; writes the block total.
+__global__ __launch_bounds__(kThreads) void k_scan_block(uint64_t *data, int64_t n,
+                                                         uint64_t *block_sums) {
+    __shared__ uint64_t wsum[kThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t x = (base + k < n) ? data[base + k] : 0;
+        run += x;
+        v[k] = run;
+    }
+    const uint64_t incl = wave_inclusive_scan(run, lane);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint64_t wave_base = 0;
+    for (uint32_t k = 0; k < w; ++k) wave_base += wsum[k];
+    const uint64_t excl = wave_base + incl - run;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) data[base + k] = v[k] + excl;
+    if (tid == kThreads - 1 && block_sums) block_sums[blockIdx.x] = excl + run;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint64_t *data, int64_t n,
+                                                       const uint64_t *block_sums) {
+    if (blockIdx.x == 0) return;
+    const uint64_t add = block_sums[blockIdx.x - 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock;
+    for (int k = threadIdx.x; k < kScanBlock; k += kThreads)
+        if (base + k < n) data[base + k] += add;
+}
+
+size_t scan_scratch_elems(int64_t n) {
+    size_t total = 0;
+    int64_t m = n;
+    while (m > kScanBlock) {
+        m = (m + kScanBlock - 1) / kScanBlock;
+        total += (size_t)m;
+    }
+    return total + 1;
+}
+
+hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kThreads), 0, stream, data, n, (uint64_t *)nullptr);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(kThreads), 0, stream, data, n, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = inclusive_scan_u64(scratch, nb, scratch + nb, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kThreads), 0, stream, data, n,
+                       (const uint64_t *)scratch);
+    return hipGetLastError();
+}
+
+__global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, uint32_t *fp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint64_t s = p ? pend[p - 1] : 0, e = pend[p];
+    if (e <= s) return;
+    for (uint64_t t = (s + tb - 1) / tb; t * tb < e; ++t) fp[t] = (uint32_t)p;
+}
+
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes,
+                              uint32_t *first_piece, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_first_piece, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pend, n,
+                       tile_bytes, first_piece);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------- wire order of the plain form
+
+constexpr int kPairChunk = 2048;  // pieces per chunk of the multi-split
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t pair_key(uint32_t r, int32_t P) {
+    const uint32_t s = r >> 16, d = r & 0xffffu;
+    return s == d ? kNoKey : s * (uint32_t)P + d;
+}
+
+// Per chunk, elements sent per (source, destination) pair -> hist[key][chunk].
+__global__ __launch_bounds__(kThreads) void k_pair_hist(const uint32_t *ranks, const piece *pieces,
+                                                        int64_t n, int32_t P, uint64_t *hist,
+                                                        int64_t n_chunks) {
+    extern __shared__ unsigned long long h[];
+    const uint32_t K = (uint32_t)P * (uint32_t)P;
+    for (uint32_t k = threadIdx.x; k < K; k += kThreads) h[k] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kPairChunk;
+    for (int i = threadIdx.x; i < kPairChunk; i += kThreads) {
+        const int64_t p = base + i;
+        if (p >= n) break;
+        const uint32_t key = pair_key(ranks[p], P);
+        if (key != kNoKey) atomicAdd(&h[key], (unsigned long long)pieces[p].rows * pieces[p].cols);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += kThreads) hist[(int64_t)k * n_chunks + blockIdx.x] = h[k];
+}
+
+// Exclusive scan over chunks per key; pair totals.
+__global__ void k_pair_scan(uint64_t *hist, int64_t n_chunks, int32_t P, uint64_t *pair_counts) {
+    const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= (int64_t)P * P) return;
+    uint64_t run = 0;
+    for (int64_t c = 0; c < n_chunks; ++c) {
+        const uint64_t v = hist[key * n_chunks + c];
+        hist[key * n_chunks + c] = run;
+        run += v;
+    }
+    pair_counts[key] = run;
+}
+
+// One wave per chunk walks its pieces 64 at a time; for every distinct pair
+// key among the 64 lanes a wavefront prefix scan of the piece sizes gives each
+// piece its offset inside the pair's segment.
+__global__ __launch_bounds__(64) void k_pair_within(const uint32_t *ranks, const piece *pieces,
+                                                    int64_t n, int32_t P, const uint64_t *hist,
+                                                    int64_t n_chunks, uint64_t *within) {
+    extern __shared__ unsigned long long run[];
+    const uint32_t K = (uint32_t)P * (uint32_t)P;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t k = lane; k < K; k += 64) run[k] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kPairChunk;
+    for (int g = 0; g < kPairChunk; g += 64) {
+        const int64_t p = base + g + lane;
+        const bool valid = p < n;
+        const uint32_t key = valid ? pair_key(ranks[p], P) : kNoKey;
+        const uint64_t el = valid ? (uint64_t)pieces[p].rows * pieces[p].cols : 0;
+        if (valid && key == kNoKey) within[p] = ~0ull;
+        unsigned long long pending = __ballot(key != kNoKey);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const uint32_t k = __shfl(key, leader, 64);
+            const bool mine = key == k;
+            const uint64_t x = mine ? el : 0;
+            const uint64_t incl = wave_inclusive_scan(x, lane);
+            const uint64_t tot = __shfl(incl, 63, 64);
+            const uint64_t b = hist[(int64_t)k * n_chunks + blockIdx.x] + run[k];
+            if (mine) within[p] = b + incl - x;
+            __syncthreads();
+            if (lane == 0) run[k] += tot;
+            __syncthreads();
+            pending &= ~__ballot(mine);
+        }
+    }
+}
+
+hipError_t launch_pair_offsets(const uint32_t *ranks, const piece *pieces, int64_t n,
+                               int32_t n_ranks, uint64_t *hist, int64_t n_chunks,
+                               uint64_t *within, uint64_t *pair_counts, hipStream_t stream) {
+    const size_t K = (size_t)n_ranks * n_ranks;
+    const size_t shm = K * sizeof(unsigned long long);
+    if (n == 0) {
+        return hipMemsetAsync(pair_counts, 0, K * sizeof(uint64_t), stream);
+    }
+    hipLaunchKernelGGL(k_pair_hist, dim3((unsigned)n_chunks), dim3(kThreads), shm, stream, ranks, pieces,
+                       n, n_ranks, hist, n_chunks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pair_scan, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, stream, hist, n_chunks,
+                       n_ranks, pair_counts);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pair_within, dim3((unsigned)n_chunks), dim3(64), shm, stream, ranks, pieces, n,
+                       n_ranks, (const uint64_t *)hist, n_chunks, within);
+    return hipGetLastError();
+}
+
+// Stage tables of the plain form. Stage A: pack remote pieces into the
+// source rank's send buffer (packed column-major, ld = piece rows, as
+// copy_block_to_buffer, reference communication_data.cpp:101-117) and copy
+// local pieces straight to the destination (copy_local_blocks, :224-252).
+// Stage C: unpack remote pieces from the destination rank's receive buffer
+// (copy_block_from_buffer, :119-130); local pieces become empty.
+__global__ void k_stage_pieces(const piece *pieces, const uint32_t *ranks, const uint64_t *within,
+                               int64_t n, int32_t P, const uint64_t *sbase, const uint64_t *sdspl,
+                               const uint64_t *rbase, const uint64_t *rdspl, uint64_t send_arena,
+                               uint64_t recv_arena, uint32_t es, piece *sa, piece *sc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const piece pc = pieces[p];
+    const uint32_t r = ranks[p];
+    const uint32_t s = r >> 16, d = r & 0xffffu;
+    piece a = pc, c = pc;
+    if (s != d) {
+        const uint64_t w = within[p];
+        const uint64_t soff = sbase[s] + sdspl[(int64_t)s * P + d] + w;
+        const uint64_t roff = rbase[d] + rdspl[(int64_t)d * P + s] + w;
+        a.dst = send_arena + soff * es;
+        a.ldd = pc.rows;
+        a.req = 0;
+        c.src = recv_arena + roff * es;
+        c.lds = pc.rows;
+        c.req = 0;
+    } else {
+        a.req = 1;
+        c.src = 0;
+        c.dst = 0;
+        c.cols = 0;
+        c.req = 0;
+    }
+    sa[p] = a;
+    sc[p] = c;
+}
+
+hipError_t launch_stage_pieces(const piece *pieces, const uint32_t *ranks, const uint64_t *within,
+                               int64_t n, int32_t n_ranks, const uint64_t *sbase,
+                               const uint64_t *sdspl, const uint64_t *rbase,
+                               const uint64_t *rdspl, uint64_t send_arena, uint64_t recv_arena,
+                               uint32_t esize, piece *stage_a, piece *stage_c,
+                               hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_pieces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pieces,
+                       ranks, within, n, n_ranks, sbase, sdspl, rbase, rdspl, send_arena, recv_arena,
+                       esize, stage_a, stage_c);
+    return hipGetLastError();
+}
+
+__global__ void k_concat(const piece *src, int64_t n, uint32_t req, piece *dst) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    piece pc = src[p];
+    pc.req = req;
+    dst[p] = pc;
+}
+
+hipError_t launch_concat_pieces(const piece *src, int64_t n, uint32_t req, piece *dst,
+                                hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_concat, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n, req,
+                       dst);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------- seeded test values
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// v(seed, i, j) as raw bits in a u32x4 (unused lanes zero).
+__device__ __forceinline__ u32x4 seeded_value(int dtype, uint64_t seed, int64_t i, int64_t j,
+                                              int64_t n_cols) {
+    const uint64_t key = (uint64_t)i * (uint64_t)n_cols + (uint64_t)j;
+    u32x4 r = {0u, 0u, 0u, 0u};
+    if (dtype == DT_F32) {
+        const uint64_t x = splitmix64(seed ^ key);
+        r.x = __float_as_uint((float)(x >> 40) * 0x1p-23f - 1.0f);
+    } else if (dtype == DT_F64) {
+        const uint64_t x = splitmix64(seed ^ key);
+        const u32x2 v = d2u((double)(x >> 11) * 0x1p-52 - 1.0);
+        r.x = v.x;
+        r.y = v.y;
+    } else if (dtype == DT_C64) {
+        const uint64_t x = splitmix64(seed ^ key);
+        r.x = __float_as_uint((float)(x >> 40) * 0x1p-23f - 1.0f);
+        r.y = __float_as_uint((float)((x >> 16) & 0xffffffull) * 0x1p-23f - 1.0f);
+    } else {
+        const uint64_t x0 = splitmix64(seed ^ (2 * key));
+        const uint64_t x1 = splitmix64(seed ^ (2 * key + 1));
+        const u32x2 v0 = d2u((double)(x0 >> 11) * 0x1p-52 - 1.0);
+        const u32x2 v1 = d2u((double)(x1 >> 11) * 0x1p-52 - 1.0);
+        r = u32x4{v0.x, v0.y, v1.x, v1.y};
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t dtype_size(int dtype) {
+    return dtype == DT_F32 ? 4u : (dtype == DT_C128 ? 16u : 8u);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill(const fill_block *blocks, int dtype,
+                                                   uint64_t seed, int64_t n_cols) {
+    const fill_block b = blocks[blockIdx.x];
+    const uint32_t es = dtype_size(dtype);
+    const int32_t rows = b.r1 - b.r0, cols = b.c1 - b.c0;
+    for (int32_t j = blockIdx.y; j < cols; j += gridDim.y) {
+        unsigned char *col = (unsigned char *)b.ptr + (uint64_t)j * b.ld * es;
+        for (int32_t i = threadIdx.x; i < rows; i += kThreads) {
+            const u32x4 v = seeded_value(dtype, seed, b.r0 + i, b.c0 + j, n_cols);
+            unsigned char *e = col + (uint64_t)i * es;
+            if (es == 4)
+                *(uint32_t *)e = v.x;
+            else if (es == 8)
+                *(u32x2 *)e = u32x2{v.x, v.y};
+            else
+                *(u32x4 *)e = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_verify(const fill_block *blocks, int dtype,
+                                                     uint64_t seed, int trans, int conj,
+                                                     int64_t a_n_cols, unsigned long long *bad) {
+    const fill_block b = blocks[blockIdx.x];
+    const uint32_t es = dtype_size(dtype);
+    const int32_t rows = b.r1 - b.r0, cols = b.c1 - b.c0;
+    unsigned long long cnt = 0;
+    for (int32_t j = blockIdx.y; j < cols; j += gridDim.y) {
+        const unsigned char *col = (const unsigned char *)b.ptr + (uint64_t)j * b.ld * es;
+        for (int32_t i = threadIdx.x; i < rows; i += kThreads) {
+            const int64_t gi = b.r0 + i, gj = b.c0 + j;
+            u32x4 w = trans ? seeded_value(dtype, seed, gj, gi, a_n_cols)
+                            : seeded_value(dtype, seed, gi, gj, a_n_cols);
+            if (conj) {
+                if (dtype == DT_C64) w.y ^= 0x80000000u;
+                if (dtype == DT_C128) w.w ^= 0x80000000u;
+            }
+            const unsigned char *e = col + (uint64_t)i * es;
+            bool ok;
+            if (es == 4)
+                ok = *(const uint32_t *)e == w.x;
+            else if (es == 8) {
+                const u32x2 g = *(const u32x2 *)e;
+                ok = g.x == w.x && g.y == w.y;
+            } else {
+                const u32x4 g = *(const u32x4 *)e;
+                ok = g.x == w.x && g.y == w.y && g.z == w.z && g.w == w.w;
+            }
+            cnt += ok ? 0ull : 1ull;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(bad, cnt);
+}
+
+hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
+                       int64_t n_cols, hipStream_t stream) {
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)n_blocks, 32), dim3(kThreads), 0, stream, blocks, dtype,
+                       seed, n_cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_verify(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
+                         int trans, int conj, int64_t a_n_cols, unsigned long long *bad,
+                         hipStream_t stream) {
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_verify, dim3((unsigned)n_blocks, 32), dim3(kThreads), 0, stream, blocks, dtype,
+                       seed, trans, conj, a_n_cols, bad);
+    return hipGetLastError();
+}
+
+}  // namespace g2g

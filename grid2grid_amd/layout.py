@@ -1,0 +1,228 @@
+"""Grid-like layouts of P virtual ranks, and builders for the ones the
+benchmarks and tests use.
+
+A `Layout` is the host description of the reference's grid_layout<T>
+(reference grid_layout.hpp:7-63) for ALL virtual ranks at once: the grid2D
+splits, owners[i][j], the rank count, and one block per grid cell given by
+its leading dimension and its element offset inside one memory arena. The
+arena can be host memory (numpy) or device memory (a torch tensor); the
+descriptor handed to the C ABI is built from the arena's base address.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import G2GBlock, G2GLayout, DTYPE_SIZE
+
+BLOCK_DTYPE = np.dtype([("row", "<i4"), ("col", "<i4"), ("stride", "<i8"), ("data", "<u8")])
+assert BLOCK_DTYPE.itemsize == ctypes.sizeof(G2GBlock)
+
+
+class Descriptor:
+    """A g2g_layout plus the arrays it points into (kept alive here)."""
+
+    def __init__(self, layout, base, esize):
+        self.rows = np.ascontiguousarray(layout.rows_split, dtype=np.int32)
+        self.cols = np.ascontiguousarray(layout.cols_split, dtype=np.int32)
+        self.owners = np.ascontiguousarray(layout.owners, dtype=np.int32)
+        blk = np.empty(layout.n_blocks, dtype=BLOCK_DTYPE)
+        blk["row"] = layout.blk_row
+        blk["col"] = layout.blk_col
+        blk["stride"] = layout.blk_stride
+        blk["data"] = np.uint64(base) + layout.blk_offset.astype(np.uint64) * np.uint64(esize)
+        self.blocks = blk
+        self.struct = G2GLayout(
+            len(self.rows), len(self.cols),
+            self.rows.ctypes.data, self.cols.ctypes.data, self.owners.ctypes.data,
+            layout.n_ranks, layout.n_blocks, self.blocks.ctypes.data if layout.n_blocks else None)
+
+    def ref(self):
+        return ctypes.byref(self.struct)
+
+
+class Layout:
+    """Splits, owners and per-cell blocks (row, col, stride, arena offset)."""
+
+    def __init__(self, rows_split, cols_split, owners, n_ranks, blk_row, blk_col, blk_stride,
+                 blk_offset, arena_elems, rank_base=None):
+        self.rows_split = np.asarray(rows_split, dtype=np.int32)
+        self.cols_split = np.asarray(cols_split, dtype=np.int32)
+        self.owners = np.asarray(owners, dtype=np.int32).reshape(
+            len(self.rows_split) - 1, len(self.cols_split) - 1)
+        self.n_ranks = int(n_ranks)
+        self.blk_row = np.asarray(blk_row, dtype=np.int32)
+        self.blk_col = np.asarray(blk_col, dtype=np.int32)
+        self.blk_stride = np.asarray(blk_stride, dtype=np.int64)
+        self.blk_offset = np.asarray(blk_offset, dtype=np.int64)
+        self.arena_elems = int(arena_elems)
+        self.rank_base = rank_base
+
+    @property
+    def n_blocks(self):
+        return len(self.blk_row)
+
+    @property
+    def m(self):
+        return int(self.rows_split[-1] - self.rows_split[0])
+
+    @property
+    def n(self):
+        return int(self.cols_split[-1] - self.cols_split[0])
+
+    def descriptor(self, base, esize):
+        return Descriptor(self, int(base), esize)
+
+    def element_mask(self):
+        """Boolean mask over the arena: True where an element of a block lives."""
+        mask = np.zeros(self.arena_elems, dtype=bool)
+        rs, cs = self.rows_split, self.cols_split
+        for r, c, ld, off in zip(self.blk_row, self.blk_col, self.blk_stride, self.blk_offset):
+            nr = rs[r + 1] - rs[r]
+            nc = cs[c + 1] - cs[c]
+            idx = off + np.arange(nc)[:, None] * ld + np.arange(nr)[None, :]
+            mask[idx.ravel()] = True
+        return mask
+
+
+# ------------------------------------------------------------ block-cyclic
+
+def line_split(begin, end, blk_len):
+    """scalapack_layout.cpp:152-177: splits of [begin, end) at block borders,
+    relative to begin."""
+    length = end - begin
+    rem = blk_len - begin % blk_len
+    splits = [0]
+    if rem >= length:
+        splits.append(length)
+        return splits
+    if rem != 0:
+        splits.append(rem)
+    for _ in range((length - rem) // blk_len):
+        splits.append(splits[-1] + blk_len)
+    if splits[-1] != length:
+        splits.append(length)
+    return splits
+
+
+def rank_from_grid(row, col, prow, pcol, ordering):
+    """scalapack_layout.cpp:40-57; ordering 'C' (column-major) or 'R'."""
+    if ordering == "C":
+        return col * prow + row
+    return row * pcol + col
+
+
+def scalapack_layout(m, n, mb, nb, prow, pcol, ordering="C", lld="padded", ia=1, ja=1,
+                     subm=None, subn=None, src=(0, 0), rank_gap=0, n_ranks=None):
+    """Block-cyclic layout of all prow*pcol ranks, as get_scalapack_grid builds
+    it on each rank (scalapack_layout.cpp:178-346).
+
+    lld: an int (the explicit-lld overload, same for every rank), "padded"
+    (the short overloads: whole owned blocks times mb, :305-335) or "rows"
+    (exactly the local rows). Rank r's local buffer (lld x local columns,
+    column-major) starts at rank_base[r] in the arena; rank_gap elements
+    separate consecutive rank buffers.
+    """
+    subm = m if subm is None else subm
+    subn = n if subn is None else subn
+    i0, j0 = ia - 1, ja - 1
+    rows_split = np.array(line_split(i0, i0 + subm, mb), dtype=np.int64)
+    cols_split = np.array(line_split(j0, j0 + subn, nb), dtype=np.int64)
+    gr, gc = len(rows_split) - 1, len(cols_split) - 1
+    brow, bcol = i0 // mb, j0 // nb
+    ssr = (brow % prow + src[0]) % prow
+    ssc = (bcol % pcol + src[1]) % pcol
+    ii = np.arange(gr)
+    jj = np.arange(gc)
+    rank_row = (ii % prow + ssr) % prow
+    rank_col = (jj % pcol + ssc) % pcol
+    if ordering == "C":
+        owners = rank_col[None, :] * prow + rank_row[:, None]
+    else:
+        owners = rank_row[:, None] * pcol + rank_col[None, :]
+    P = prow * pcol
+    # local coordinates of each block inside its rank's buffer (:244-260)
+    blk_loc_row = (brow + ii) // prow
+    blk_loc_col = (bcol + jj) // pcol
+    sub_loc_row = i0 + rows_split[:-1] - (brow + ii) * mb
+    sub_loc_col = j0 + cols_split[:-1] - (bcol + jj) * nb
+    row_off = blk_loc_row * mb + sub_loc_row          # per block row
+    col_off = blk_loc_col * nb + sub_loc_col          # per block col
+    row_end = row_off + np.diff(rows_split)
+    col_end = col_off + np.diff(cols_split)
+    # per rank: local rows/cols used
+    lld_r = np.zeros(P, dtype=np.int64)
+    lcols_r = np.zeros(P, dtype=np.int64)
+    for pr in range(prow):
+        rows_used = row_end[rank_row == pr].max() if np.any(rank_row == pr) else 0
+        for pc in range(pcol):
+            r = rank_from_grid(pr, pc, prow, pcol, ordering)
+            cols_used = col_end[rank_col == pc].max() if np.any(rank_col == pc) else 0
+            if isinstance(lld, (int, np.integer)):
+                lld_r[r] = int(lld)
+            elif lld == "rows":
+                lld_r[r] = max(rows_used, 1)
+            else:  # padded: whole owned blocks (:319-323)
+                n_blk_rows = int(np.ceil(m / mb))
+                own = n_blk_rows // prow + (1 if pr < n_blk_rows % prow else 0)
+                lld_r[r] = max(own * mb, 1)
+            lcols_r[r] = cols_used
+    size_r = lld_r * lcols_r
+    rank_base = np.zeros(P, dtype=np.int64)
+    acc = 0
+    for r in range(P):
+        rank_base[r] = acc
+        acc += int(size_r[r]) + int(rank_gap)
+    own = owners.ravel()
+    I, J = np.meshgrid(ii, jj, indexing="ij")
+    I, J = I.ravel(), J.ravel()
+    strides = lld_r[own]
+    offsets = rank_base[own] + row_off[I] + strides * col_off[J]
+    return Layout(rows_split, cols_split, owners, n_ranks or P, I, J, strides, offsets, acc,
+                  rank_base=rank_base)
+
+
+# ------------------------------------------------------------- arbitrary
+
+def random_splits(rng, total, count, lo, hi):
+    """count block sizes drawn uniform in [lo, hi], rescaled to sum to total
+    (remainder on the last block)."""
+    sizes = rng.integers(lo, hi + 1, size=count).astype(np.float64)
+    sizes = np.maximum(1, np.floor(sizes * total / sizes.sum())).astype(np.int64)
+    sizes[-1] += total - sizes.sum()
+    if sizes[-1] < 1:
+        raise ValueError("split rescaling produced an empty block")
+    return np.concatenate([[0], np.cumsum(sizes)])
+
+
+def arbitrary_layout(rng, m, n, n_row_blocks, n_col_blocks, n_ranks, lo=200, hi=1200,
+                     pad=(0, 63), gap=(0, 64), rows_split=None, cols_split=None):
+    """Irregular grid-like layout: random splits, owners uniform over ranks,
+    per-block ld = rows + U[pad], blocks of each rank placed in shuffled
+    order with random gaps in one arena."""
+    rs = random_splits(rng, m, n_row_blocks, lo, hi) if rows_split is None else np.asarray(rows_split)
+    cs = random_splits(rng, n, n_col_blocks, lo, hi) if cols_split is None else np.asarray(cols_split)
+    gr, gc = len(rs) - 1, len(cs) - 1
+    owners = rng.integers(0, n_ranks, size=(gr, gc))
+    I, J = np.meshgrid(np.arange(gr), np.arange(gc), indexing="ij")
+    I, J = I.ravel(), J.ravel()
+    rows = np.diff(rs)[I]
+    cols = np.diff(cs)[J]
+    ld = rows + rng.integers(pad[0], pad[1] + 1, size=len(I))
+    sizes = ld * cols
+    order = rng.permutation(len(I))
+    gaps = rng.integers(gap[0], gap[1] + 1, size=len(I))
+    offsets = np.zeros(len(I), dtype=np.int64)
+    acc = 0
+    for k in order:  # rank-agnostic shuffled placement
+        acc += int(gaps[k])
+        offsets[k] = acc
+        acc += int(sizes[k])
+    return Layout(rs, cs, owners, n_ranks, I, J, ld, offsets, acc + int(gap[1]))
+
+
+def transposed_shape(layout):
+    return layout.n, layout.m
+
+
+def esize(dtype):
+    return DTYPE_SIZE[dtype]

@@ -1,0 +1,209 @@
+/*
+ * grid2grid_amd.h -- C ABI of the MI355X-native grid2grid::transform.
+ *
+ * The reference redistributes a matrix between two grid-like layouts over P
+ * MPI ranks (src/grid2grid/transform.hpp:13-43). Here the P ranks are P
+ * virtual ranks on one GPU: every block of every rank is a device buffer
+ * (arbitrary leading dimension, arbitrary placement), and the MPI exchange is
+ * an on-device data movement. The entry points are plain C (pointers, sizes,
+ * status codes) so any host language can bind them; the headers under include/grid2grid are
+ * the header-only C++ facade with the reference's own type and function names.
+ *
+ * Conventions
+ *  - Layouts are column-major inside each block (reference block.hpp:59).
+ *  - A layout describes the whole matrix: its blocks array holds the blocks of
+ *    every virtual rank; owners[i][j] decides which rank owns cell (i, j).
+ *    Every cell must have exactly one block.
+ *  - Splits are int32 (the reference uses int); element offsets are 64-bit.
+ *  - Streams are hipStream_t passed as void* (NULL = the null stream).
+ *  - Every function returns G2G_OK (0) or an error code; g2g_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - Results are bit-exact with the reference: elements move as bit patterns
+ *    (no float conversion), 'C' flips the sign bit of the imaginary part, and
+ *    the scaled form computes beta*dst + alpha*op(src) without FMA contraction.
+ */
+#ifndef GRID2GRID_AMD_H
+#define GRID2GRID_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G2G_ABI_VERSION 1
+
+/* Element types; the reference instantiates transform for these four
+ * (src/grid2grid/transform.cpp:206-298). */
+typedef enum g2g_dtype {
+    G2G_FLOAT32 = 0,    /* float                */
+    G2G_FLOAT64 = 1,    /* double               */
+    G2G_COMPLEX64 = 2,  /* std::complex<float>  */
+    G2G_COMPLEX128 = 3  /* std::complex<double> */
+} g2g_dtype;
+
+typedef enum g2g_status {
+    G2G_OK = 0,
+    G2G_ERR_INVALID = 1,     /* bad argument or inconsistent layouts (reference: std::runtime_error) */
+    G2G_ERR_HIP = 2,         /* a HIP runtime call failed */
+    G2G_ERR_NOMEM = 3,       /* device or host allocation failed */
+    G2G_ERR_BUDGET = 4,      /* request rejected by the HBM admission budget */
+    G2G_ERR_UNSUPPORTED = 5, /* valid request outside what this build supports */
+    G2G_ERR_STATE = 6        /* object used in the wrong state */
+} g2g_status;
+
+/* One block of a layout: cell (row, col) of the grid, stored column-major at
+ * `data` (device pointer) with leading dimension `stride` (elements).
+ * Mirrors grid2grid::block<T> (reference block.hpp:60-146): coordinates,
+ * data pointer and stride; the intervals follow from the grid. */
+typedef struct g2g_block {
+    int32_t row;
+    int32_t col;
+    int64_t stride;
+    void *data;
+} g2g_block;
+
+/* A grid-like layout: grid2D splits (reference grid2D.hpp:17-39), owners and
+ * rank count (assigned_grid2D, grid2D.hpp:46-125) and the local blocks of all
+ * virtual ranks (local_blocks<T>, block.hpp:167-190). */
+typedef struct g2g_layout {
+    int32_t n_rows_split;      /* entries of rows_split = block rows + 1 */
+    int32_t n_cols_split;      /* entries of cols_split = block cols + 1 */
+    const int32_t *rows_split; /* strictly increasing */
+    const int32_t *cols_split; /* strictly increasing */
+    const int32_t *owners;     /* row-major, (n_rows_split-1) x (n_cols_split-1) */
+    int32_t n_ranks;           /* virtual ranks of the layout */
+    int32_t n_blocks;          /* entries of blocks */
+    const g2g_block *blocks;
+} g2g_layout;
+
+typedef struct g2g_plan_s *g2g_plan;
+
+/* Summary of a plan. */
+typedef struct g2g_plan_info {
+    int64_t m, n;            /* final matrix rows and columns */
+    int64_t n_pieces;        /* intersections of initial and final cells */
+    int64_t bytes;           /* matrix bytes moved (m*n*sizeof(T)) */
+    int64_t remote_bytes;    /* bytes whose source and destination ranks differ */
+    int64_t n_tiles;         /* work items of the fused gather */
+    int64_t tile_bytes;      /* bytes per work item */
+    int32_t n_ranks;         /* max(P_initial, P_final) */
+    int32_t dtype;
+    int32_t trans;           /* 'N', 'T' or 'C' */
+    int32_t reserved;
+} g2g_plan_info;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *g2g_last_error(void);
+
+/* Library version / build identification. */
+int g2g_version(void);
+
+/* Build a plan for transforming `initial` into `final`, as
+ * grid2grid::transform(initial, final, trans, alpha, beta, comm) does
+ * (reference transform.cpp:126-164). The planning of the reference --
+ * the grid cover (grid_cover.cpp:54-121), the decomposition into pieces
+ * (utils.hpp:10-89), the sort by rank and the per-rank counts, displacements
+ * and offsets (utils.hpp:141-212, communication_data.cpp:32-99) -- happens
+ * here, once per layout pair: the cover is computed on the host and uploaded
+ * as an interval table, pieces and offsets are produced on the device.
+ * trans: 'N', 'T' or 'C' (final holds A, A^T or A^H; `initial` describes A in
+ * its storage orientation, `final` describes op(A)). Unlike the reference the
+ * caller's `initial` descriptor is not modified.
+ * Planning work is ordered on `stream`; the call returns after it finished.
+ * The plan keeps the block pointers; the buffers must outlive the plan. */
+int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final, int dtype, char trans,
+                    void *stream, g2g_plan *out);
+
+int g2g_plan_destroy(g2g_plan plan);
+
+int g2g_plan_get_info(g2g_plan plan, g2g_plan_info *info);
+
+/* Run the transform: fused gather (pack, exchange and unpack in one kernel).
+ * Replaces exchange_async (reference transform.cpp:41-124) and the copy
+ * kernels it calls (communication_data.cpp:101-252, memory_utils.hpp:17-309).
+ * alpha/beta point to one host element of the plan's dtype (NULL, NULL =
+ * 1 and 0 = plain copy). Every element of every final block is written;
+ * bytes outside the pieces (ld padding, gaps) are never touched.
+ * Asynchronous on `stream`. */
+int g2g_transform(g2g_plan plan, const void *alpha, const void *beta, void *stream);
+
+/* The plain form, mirroring the reference step for step: pack every remote
+ * piece into the per-rank send buffers in the reference's wire order
+ * (communication_data.cpp:132-142) together with the local copies
+ * (communication_data.cpp:224-252), exchange each (source, destination)
+ * segment between virtual-rank buffers (transform.cpp:53-87), then unpack
+ * (communication_data.cpp:157-173). Staging buffers are allocated by the
+ * plan on first use. Asynchronous on `stream`. */
+int g2g_transform_staged(g2g_plan plan, const void *alpha, const void *beta, void *stream);
+
+/* Per-rank send counts of the plan: counts[s * P + d] = elements rank s sends
+ * to rank d (0 on the diagonal), as communication_data::counts
+ * (communication_data.cpp:72-78). `P` is the plan's n_ranks. */
+int g2g_plan_counts(g2g_plan plan, int64_t *counts, int64_t capacity);
+
+/* Copy of the staged form's wire layout, for inspection: per piece (in the
+ * plan's piece order) its global row/col origin, shape, source and destination
+ * ranks and element offsets in the source rank's send buffer and the
+ * destination rank's receive buffer (-1 for local pieces). Arrays hold
+ * n_pieces entries; any pointer may be NULL. Synchronous. */
+int g2g_plan_wire(g2g_plan plan, int32_t *row0, int32_t *col0, int32_t *rows, int32_t *cols,
+                  int32_t *src_rank, int32_t *dst_rank, int64_t *send_off, int64_t *recv_off,
+                  int64_t capacity);
+
+/* Staging buffers of the plain form after g2g_transform_staged: device
+ * pointer of rank r's packed send buffer (reference communication_data::buffer
+ * of the send side) and its size in elements. Synchronous. */
+int g2g_plan_send_buffer(g2g_plan plan, int rank, const void **data, int64_t *elems);
+
+/* ---------------------------------------------------------------- batches */
+
+typedef struct g2g_batch_s *g2g_batch;
+
+/* Group several plans (any mix of dtypes and 'N'/'T'/'C') so that they run
+ * in ONE persistent launch. Counterpart of the reference's vector overloads
+ * and transformer<T> (transform.cpp:166-204, transformer.hpp:8-62), without
+ * their single-T restriction. alphas[k]/betas[k] point to one element of plan
+ * k's dtype, or the arrays are NULL (plain copies). Plans must outlive the
+ * batch. */
+int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
+                     const void *const *betas, void *stream, g2g_batch *out);
+
+/* Run every transform of the batch in one persistent kernel that pulls
+ * (transform, tile) work items from a device queue. */
+int g2g_batch_execute(g2g_batch batch, void *stream);
+
+/* Same work replayed from a hipGraph captured on first use (the queue reset
+ * is a kernel node, not a memset). */
+int g2g_batch_execute_graph(g2g_batch batch, void *stream);
+
+/* Bytes moved by one execution of the batch. */
+int g2g_batch_bytes(g2g_batch batch, int64_t *bytes);
+
+int g2g_batch_destroy(g2g_batch batch);
+
+/* ------------------------------------------------------------ tuning */
+
+/* Work-item size (bytes) and launch mode used for plans created afterwards:
+ * tile_bytes in [4096, 1<<26] (0 = automatic), persistent 0/1 (-1 = automatic). */
+int g2g_set_tuning(int64_t tile_bytes, int persistent);
+
+/* ------------------------------------------------ test-support kernels */
+
+/* Fill every block of `layout` with the seeded value v(seed, i, j) of its
+ * global element (i, j), i.e. splitmix64(seed ^ (i * n_cols + j)) mapped to
+ * [-1, 1) exactly (f64: (x>>11)*2^-52-1; f32: (x>>40)*2^-23-1; c64: real from
+ * x>>40, imaginary from (x>>16)&0xffffff; c128: real/imaginary from keys 2k
+ * and 2k+1), n_cols = the layout's global columns. Asynchronous. */
+int g2g_fill_layout(const g2g_layout *layout, int dtype, uint64_t seed, void *stream);
+
+/* Count elements of `final` that differ bitwise from op(A) where A(i, j) =
+ * v(seed, i, j) with a_n_cols columns. Synchronous; result in *mismatches. */
+int g2g_verify_layout(const g2g_layout *final, int dtype, uint64_t seed, char trans,
+                      int64_t a_n_cols, void *stream, int64_t *mismatches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRID2GRID_AMD_H */

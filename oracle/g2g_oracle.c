@@ -1,0 +1,781 @@
+/*
+ * g2g_oracle.c -- CPU restatement of grid2grid::transform (TEST INFRASTRUCTURE).
+ *
+ * Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg, as the checker. The product never links or loads it.
+ *
+ * The restatement follows the reference step by step (file:line cited per
+ * function, paths relative to the reference root):
+ *   cover      grid_cover.cpp:54-121
+ *   decompose  utils.hpp:10-89, block.cpp:165-196 (subblock), block.cpp:249-288
+ *   sort       utils.hpp:141-144, communication_data.cpp:26-30, block.cpp:207-225
+ *   plan       communication_data.cpp:51-99
+ *   pack       communication_data.cpp:101-142, memory_utils.hpp:41-184
+ *   exchange   transform.cpp:41-124 (MPI_Isend/Irecv become a memcpy per segment)
+ *   local      communication_data.cpp:197-252
+ *   unpack     communication_data.cpp:119-173, memory_utils.hpp:79-122
+ *   scaling    memory_utils.hpp:25-39 (dst = beta*dst + alpha*src, no FMA)
+ *
+ * Build with -ffp-contract=off: the reference's x86-64 Release build has no
+ * FMA, so beta*dst + alpha*src is round(round(beta*dst) + round(alpha*src)),
+ * and complex products are (ac - bd, ad + bc) for finite operands.
+ */
+#include "g2g_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+size_t orc_dtype_size(int dtype) {
+    switch (dtype) {
+    case ORC_F32: return 4;
+    case ORC_F64: return 8;
+    case ORC_C64: return 8;
+    case ORC_C128: return 16;
+    default: return 0;
+    }
+}
+
+static int is_complex(int dtype) { return dtype == ORC_C64 || dtype == ORC_C128; }
+
+/* ------------------------------------------------------------------ cover */
+
+/* grid_cover.cpp:54-121, statement by statement (including the way i_prev
+ * walks back after overshooting). */
+int orc_decomp_cover(const int32_t *blue, int n_blue, const int32_t *red, int n_red,
+                     int32_t *out) {
+    if (n_blue < 1 || n_red < 1) return -1;
+    if (blue[n_blue - 1] != red[n_red - 1]) return -1; /* the reference's assert (:70) */
+    int r = red[0];
+    int r_prev = red[0];
+    int i_red = 1;
+    int i_prev = 0;
+    int i_blue = 1;
+    int k = 0;
+    (void)r;
+    while (i_blue < n_blue) {
+        int r_left = r_prev;
+        if (i_blue > 1) {
+            while (r_left < blue[i_blue - 1]) {
+                i_prev++;
+                if (i_prev >= n_red) return -1;
+                r_left = red[i_prev];
+            }
+            if (r_left > blue[i_blue - 1]) {
+                i_prev--;
+                r_left = red[i_prev];
+            }
+        }
+        int i_left = i_prev;
+        int b = blue[i_blue];
+        if (i_red >= n_red) return -1;
+        r = red[i_red];
+        while (r < b) {
+            r_prev = r;
+            i_prev = i_red;
+            i_red++;
+            if (i_red >= n_red) return -1;
+            r = red[i_red];
+        }
+        out[2 * k] = i_left;
+        out[2 * k + 1] = i_red;
+        k++;
+        i_blue++;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------- working layouts */
+
+/* A grid_layout after transpose_or_conjugate (grid_layout.hpp:18-25):
+ * splits swapped, owners transposed (grid2D.cpp:47-50,93-110), blocks
+ * flagged (block.cpp:249-288). Intervals are kept in the working
+ * orientation; data/stride stay those of the storage. */
+typedef struct {
+    int n_rs, n_cs;
+    const int32_t *rs, *cs;
+    const orc_layout *src; /* original descriptor */
+    int transposed;        /* working orientation = transpose of storage */
+    int conjugated;
+} wlayout;
+
+static void wl_init(wlayout *w, const orc_layout *l, char trans) {
+    int t = (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c');
+    w->src = l;
+    w->transposed = t;
+    w->conjugated = (trans == 'C' || trans == 'c');
+    if (t) {
+        w->n_rs = l->n_cols_split;
+        w->n_cs = l->n_rows_split;
+        w->rs = l->cols_split;
+        w->cs = l->rows_split;
+    } else {
+        w->n_rs = l->n_rows_split;
+        w->n_cs = l->n_cols_split;
+        w->rs = l->rows_split;
+        w->cs = l->cols_split;
+    }
+}
+
+/* assigned_grid2D::owner (grid2D.cpp:65-67) in working orientation. */
+static int wl_owner(const wlayout *w, int i, int j) {
+    const orc_layout *l = w->src;
+    int ncols = l->n_cols_split - 1;
+    if (w->transposed) return l->owners[(int64_t)j * ncols + i];
+    return l->owners[(int64_t)i * ncols + j];
+}
+
+/* Block b's coordinates and intervals in working orientation. */
+static void wl_block(const wlayout *w, int b, int *row, int *col, int *r0, int *r1, int *c0,
+                     int *c1) {
+    const orc_block *blk = &w->src->blocks[b];
+    int br = blk->row, bc = blk->col;
+    if (w->transposed) {
+        int t = br;
+        br = bc;
+        bc = t;
+    }
+    *row = br;
+    *col = bc;
+    *r0 = w->rs[br];
+    *r1 = w->rs[br + 1];
+    *c0 = w->cs[bc];
+    *c1 = w->cs[bc + 1];
+}
+
+/* --------------------------------------------------------- decompose */
+
+typedef struct {
+    orc_msg *v;
+    int64_t n, cap;
+} msgvec;
+
+static int mv_push(msgvec *m, const orc_msg *x) {
+    if (m->n == m->cap) {
+        int64_t nc = m->cap ? m->cap * 2 : 1024;
+        orc_msg *nv = (orc_msg *)realloc(m->v, (size_t)nc * sizeof(orc_msg));
+        if (!nv) return -1;
+        m->v = nv;
+        m->cap = nc;
+    }
+    m->v[m->n++] = *x;
+    return 0;
+}
+
+/* utils.hpp:10-65 (decompose_block) for every block of `from` owned by
+ * `rank` (utils.hpp:67-89), against grid `to`, cover from grid_cover.hpp:44-47.
+ * The piece pointer follows block::subblock (block.cpp:165-196). */
+static int decompose_rank(const wlayout *from, const wlayout *to, int rank, int tag,
+                          size_t es, msgvec *out) {
+    int nbr = from->n_rs - 1, nbc = from->n_cs - 1;
+    int32_t *rcov = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(nbr > 0 ? nbr : 1));
+    int32_t *ccov = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(nbc > 0 ? nbc : 1));
+    int rc = -1;
+    if (!rcov || !ccov) goto done;
+    if (orc_decomp_cover(from->rs, from->n_rs, to->rs, to->n_rs, rcov)) goto done;
+    if (orc_decomp_cover(from->cs, from->n_cs, to->cs, to->n_cs, ccov)) goto done;
+    for (int b = 0; b < from->src->n_blocks; ++b) {
+        int brow, bcol, R0, R1, C0, C1;
+        wl_block(from, b, &brow, &bcol, &R0, &R1, &C0, &C1);
+        if (wl_owner(from, brow, bcol) != rank) continue;
+        if (brow < 0 || brow >= nbr || bcol < 0 || bcol >= nbc) goto done;
+        int row_first = rcov[2 * brow], row_last = rcov[2 * brow + 1];
+        int col_first = ccov[2 * bcol], col_last = ccov[2 * bcol + 1];
+        int64_t stride = from->src->blocks[b].stride;
+        int row_start = R0;
+        for (int i = row_first; i < row_last; ++i) {
+            int row_end = to->rs[i + 1] < R1 ? to->rs[i + 1] : R1;
+            int col_start = C0;
+            for (int j = col_first; j < col_last; ++j) {
+                int col_end = to->cs[j + 1] < C1 ? to->cs[j + 1] : C1;
+                orc_msg m;
+                memset(&m, 0, sizeof m);
+                m.rank = wl_owner(to, i, j);
+                m.tag = tag;
+                m.r0 = row_start;
+                m.r1 = row_end;
+                m.c0 = col_start;
+                m.c1 = col_end;
+                m.offset = -1;
+                m.stride = stride;
+                m.block_index = b;
+                m.transposed = from->transposed;
+                m.conjugated = from->conjugated;
+                if (!from->transposed) {
+                    /* data + (c.start - C0)*stride + (r.start - R0) */
+                    m.data_off = ((int64_t)(col_start - C0) * stride + (row_start - R0)) * (int64_t)es;
+                } else {
+                    /* ranges swapped first (block.cpp:180-186): storage rows are
+                     * working columns, storage columns are working rows */
+                    m.data_off = ((int64_t)(row_start - R0) * stride + (col_start - C0)) * (int64_t)es;
+                }
+                if (m.r1 > m.r0 && m.c1 > m.c0) {
+                    if (mv_push(out, &m)) goto done;
+                }
+                col_start = col_end;
+            }
+            row_start = row_end;
+        }
+    }
+    rc = 0;
+done:
+    free(rcov);
+    free(ccov);
+    return rc;
+}
+
+/* interval::operator< (interval.cpp:48-51) */
+static int ival_less(int s0, int e0, int s1, int e1) {
+    return s0 < s1 || (s0 == s1 && e0 < e1);
+}
+
+/* message::operator< (communication_data.cpp:26-30) over block::operator<
+ * (block.cpp:207-225): rank, then cols interval, then rows interval, then tag. */
+static int msg_cmp(const void *pa, const void *pb) {
+    const orc_msg *a = (const orc_msg *)pa, *b = (const orc_msg *)pb;
+    if (a->rank != b->rank) return a->rank < b->rank ? -1 : 1;
+    int cols_less = ival_less(a->c0, a->c1, b->c0, b->c1);
+    int cols_greater = ival_less(b->c0, b->c1, a->c0, a->c1);
+    if (cols_less) return -1;
+    if (cols_greater) return 1;
+    int rows_less = ival_less(a->r0, a->r1, b->r0, b->r1);
+    int rows_greater = ival_less(b->r0, b->r1, a->r0, a->r1);
+    if (rows_less) return -1;
+    if (rows_greater) return 1;
+    if (a->tag != b->tag) return a->tag < b->tag ? -1 : 1;
+    return 0;
+}
+
+int orc_n_ranks(const orc_layout *init, const orc_layout *fin) {
+    return init->n_ranks > fin->n_ranks ? init->n_ranks : fin->n_ranks;
+}
+
+/* communication_data.cpp:51-99: split remote from local, counts, dspls,
+ * offsets (remote pieces only, in sort order). */
+static void comm_data(orc_msg *m, int64_t n, int my_rank, int n_ranks, int64_t *counts,
+                      int64_t *dspls, int64_t *total) {
+    for (int r = 0; r < n_ranks; ++r) counts[r] = 0;
+    int64_t offset = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t sz = (int64_t)(m[i].r1 - m[i].r0) * (m[i].c1 - m[i].c0);
+        if (m[i].rank != my_rank) {
+            m[i].offset = offset;
+            offset += sz;
+            counts[m[i].rank] += sz;
+        } else {
+            m[i].offset = -1;
+        }
+    }
+    dspls[0] = 0;
+    for (int r = 1; r < n_ranks; ++r) dspls[r] = dspls[r - 1] + counts[r - 1];
+    *total = offset;
+}
+
+/* prepare_to_send / prepare_to_recv (utils.hpp:146-161, 183-192). */
+static int build_side(const orc_layout *init, const orc_layout *fin, char trans, int rank,
+                      int side, size_t es, msgvec *mv) {
+    wlayout wi, wf;
+    wl_init(&wi, init, trans);
+    wl_init(&wf, fin, 'N');
+    int rc = side == 0 ? decompose_rank(&wi, &wf, rank, 0, es, mv)
+                       : decompose_rank(&wf, &wi, rank, 0, es, mv);
+    if (rc) return rc;
+    qsort(mv->v, (size_t)mv->n, sizeof(orc_msg), msg_cmp);
+    return 0;
+}
+
+int64_t orc_plan_rank(const orc_layout *init, const orc_layout *fin, int dtype, char trans,
+                      int rank, int side, orc_msg *msgs, int64_t max_msgs, int64_t *counts,
+                      int64_t *dspls) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    msgvec mv = {0, 0, 0};
+    if (build_side(init, fin, trans, rank, side, es, &mv)) {
+        free(mv.v);
+        return -1;
+    }
+    int P = orc_n_ranks(init, fin);
+    int64_t total;
+    comm_data(mv.v, mv.n, rank, P, counts, dspls, &total);
+    int64_t n = mv.n;
+    if (msgs) {
+        if (n > max_msgs) {
+            free(mv.v);
+            return -1;
+        }
+        memcpy(msgs, mv.v, (size_t)n * sizeof(orc_msg));
+    }
+    free(mv.v);
+    return n;
+}
+
+/* ----------------------------------------------------- element copies */
+
+static void conj_el(int dtype, unsigned char *el) {
+    /* std::conj flips the sign of the imaginary part (block.cpp:15-21);
+     * conjugate() of real types is the identity (block.cpp:7-13). */
+    if (dtype == ORC_C64) el[7] ^= 0x80;
+    if (dtype == ORC_C128) el[15] ^= 0x80;
+}
+
+/* dst = beta*dst + alpha*src for one element (memory_utils.hpp:37). */
+static void scale_el(int dtype, unsigned char *dst, const unsigned char *src, const void *alpha,
+                     const void *beta) {
+    switch (dtype) {
+    case ORC_F32: {
+        float a = *(const float *)alpha, b = *(const float *)beta, s, d;
+        memcpy(&s, src, 4);
+        memcpy(&d, dst, 4);
+        float bd = b * d;
+        float as = a * s;
+        float r = bd + as;
+        memcpy(dst, &r, 4);
+        break;
+    }
+    case ORC_F64: {
+        double a = *(const double *)alpha, b = *(const double *)beta, s, d;
+        memcpy(&s, src, 8);
+        memcpy(&d, dst, 8);
+        double bd = b * d;
+        double as = a * s;
+        double r = bd + as;
+        memcpy(dst, &r, 8);
+        break;
+    }
+    case ORC_C64: {
+        const float *a = (const float *)alpha, *b = (const float *)beta;
+        float s[2], d[2];
+        memcpy(s, src, 8);
+        memcpy(d, dst, 8);
+        /* complex product (ac - bd, ad + bc) */
+        float bd_re = b[0] * d[0] - b[1] * d[1];
+        float bd_im = b[0] * d[1] + b[1] * d[0];
+        float as_re = a[0] * s[0] - a[1] * s[1];
+        float as_im = a[0] * s[1] + a[1] * s[0];
+        float r[2] = {bd_re + as_re, bd_im + as_im};
+        memcpy(dst, r, 8);
+        break;
+    }
+    case ORC_C128: {
+        const double *a = (const double *)alpha, *b = (const double *)beta;
+        double s[2], d[2];
+        memcpy(s, src, 16);
+        memcpy(d, dst, 16);
+        double bd_re = b[0] * d[0] - b[1] * d[1];
+        double bd_im = b[0] * d[1] + b[1] * d[0];
+        double as_re = a[0] * s[0] - a[1] * s[1];
+        double as_im = a[0] * s[1] + a[1] * s[0];
+        double r[2] = {bd_re + as_re, bd_im + as_im};
+        memcpy(dst, r, 16);
+        break;
+    }
+    }
+}
+
+/* alpha == T{1} && beta == T{0}: the exact-copy shortcut (memory_utils.hpp:32-35). */
+static int is_plain_copy(int dtype, const void *alpha, const void *beta) {
+    if (!alpha && !beta) return 1;
+    switch (dtype) {
+    case ORC_F32: return *(const float *)alpha == 1.0f && *(const float *)beta == 0.0f;
+    case ORC_F64: return *(const double *)alpha == 1.0 && *(const double *)beta == 0.0;
+    case ORC_C64: {
+        const float *a = (const float *)alpha, *b = (const float *)beta;
+        return a[0] == 1.0f && a[1] == 0.0f && b[0] == 0.0f && b[1] == 0.0f;
+    }
+    case ORC_C128: {
+        const double *a = (const double *)alpha, *b = (const double *)beta;
+        return a[0] == 1.0 && a[1] == 0.0 && b[0] == 0.0 && b[1] == 0.0;
+    }
+    }
+    return 1;
+}
+
+/* memory::copy2D / copy2D_and_scale (memory_utils.hpp:41-122), column-major. */
+static void copy2d(int dtype, size_t es, int64_t rows, int64_t cols, const unsigned char *src,
+                   int64_t ld_src, unsigned char *dst, int64_t ld_dst, int plain,
+                   const void *alpha, const void *beta) {
+    for (int64_t c = 0; c < cols; ++c) {
+        const unsigned char *s = src + (size_t)(c * ld_src) * es;
+        unsigned char *d = dst + (size_t)(c * ld_dst) * es;
+        if (plain) {
+            memcpy(d, s, (size_t)rows * es);
+        } else {
+            for (int64_t r = 0; r < rows; ++r) scale_el(dtype, d + r * es, s + r * es, alpha, beta);
+        }
+    }
+}
+
+/* memory::copy_and_transpose / copy_transpose_and_scale (memory_utils.hpp:124-256):
+ * dest[i*dest_stride + j] = op(src[j*src_stride + i]) for storage rows i <
+ * n_rows and storage cols j < n_cols. The reference's tiling and diagonal
+ * staging buffer only change the visiting order, not the result. */
+static void copy_transpose(int dtype, size_t es, int64_t n_rows, int64_t n_cols,
+                           const unsigned char *src, int64_t src_stride, unsigned char *dst,
+                           int64_t dst_stride, int conj, int plain, const void *alpha,
+                           const void *beta) {
+    unsigned char el[16];
+    for (int64_t i = 0; i < n_rows; ++i) {
+        for (int64_t j = 0; j < n_cols; ++j) {
+            memcpy(el, src + (size_t)(j * src_stride + i) * es, es);
+            if (conj) conj_el(dtype, el);
+            unsigned char *d = dst + (size_t)(i * dst_stride + j) * es;
+            if (plain)
+                memcpy(d, el, es);
+            else
+                scale_el(dtype, d, el, alpha, beta);
+        }
+    }
+}
+
+static unsigned char *msg_ptr(const orc_layout *l, const orc_msg *m) {
+    return (unsigned char *)l->blocks[m->block_index].data + m->data_off;
+}
+
+/* copy_block_to_buffer (communication_data.cpp:101-117) */
+static void pack_msg(int dtype, size_t es, const orc_layout *l, const orc_msg *m,
+                     unsigned char *dest) {
+    int64_t nr = m->r1 - m->r0, nc = m->c1 - m->c0;
+    const unsigned char *src = msg_ptr(l, m);
+    if (!m->transposed)
+        copy2d(dtype, es, nr, nc, src, m->stride, dest, nr, 1, 0, 0);
+    else
+        copy_transpose(dtype, es, nc, nr, src, m->stride, dest, nr, m->conjugated, 1, 0, 0);
+}
+
+int64_t orc_pack_rank(const orc_layout *init, const orc_layout *fin, int dtype, char trans,
+                      int rank, void *buffer, int64_t buffer_elems) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    msgvec mv = {0, 0, 0};
+    if (build_side(init, fin, trans, rank, 0, es, &mv)) {
+        free(mv.v);
+        return -1;
+    }
+    int P = orc_n_ranks(init, fin);
+    int64_t *counts = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+    int64_t *dspls = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+    int64_t total = 0;
+    comm_data(mv.v, mv.n, rank, P, counts, dspls, &total);
+    int64_t rc = total;
+    if (total > buffer_elems) {
+        rc = -1;
+    } else {
+        for (int64_t i = 0; i < mv.n; ++i) {
+            if (mv.v[i].offset < 0) continue;
+            pack_msg(dtype, es, init, &mv.v[i], (unsigned char *)buffer + (size_t)mv.v[i].offset * es);
+        }
+    }
+    free(counts);
+    free(dspls);
+    free(mv.v);
+    return rc;
+}
+
+/* ------------------------------------------------------------ transform */
+
+typedef struct {
+    msgvec send, recv;
+    int64_t *scounts, *sdspls, *rcounts, *rdspls;
+    int64_t stotal, rtotal;
+    unsigned char *sbuf, *rbuf;
+} rank_state;
+
+/* Runs exchange_async (transform.cpp:41-124) for every virtual rank; pairs
+ * index `tag` into the arrays of layouts. */
+static int run_pipeline(int n_pairs, const orc_layout *inits, const orc_layout *fins, int dtype,
+                        const char *trans, const unsigned char *alphas,
+                        const unsigned char *betas) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    int P = 0;
+    for (int k = 0; k < n_pairs; ++k) {
+        int p = orc_n_ranks(&inits[k], &fins[k]);
+        if (p > P) P = p;
+    }
+    if (P <= 0) return -1;
+    rank_state *st = (rank_state *)calloc((size_t)P, sizeof(rank_state));
+    if (!st) return -1;
+    int rc = -1;
+    /* prepare_to_send / prepare_to_recv for each rank (vector overloads
+     * utils.hpp:163-212: pieces tagged with the pair index, one sort). */
+    for (int r = 0; r < P; ++r) {
+        for (int k = 0; k < n_pairs; ++k) {
+            wlayout wi, wf;
+            wl_init(&wi, &inits[k], trans[k]);
+            wl_init(&wf, &fins[k], 'N');
+            if (decompose_rank(&wi, &wf, r, k, es, &st[r].send)) goto done;
+            if (decompose_rank(&wf, &wi, r, k, es, &st[r].recv)) goto done;
+        }
+        qsort(st[r].send.v, (size_t)st[r].send.n, sizeof(orc_msg), msg_cmp);
+        qsort(st[r].recv.v, (size_t)st[r].recv.n, sizeof(orc_msg), msg_cmp);
+        st[r].scounts = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+        st[r].sdspls = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+        st[r].rcounts = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+        st[r].rdspls = (int64_t *)calloc((size_t)P, sizeof(int64_t));
+        if (!st[r].scounts || !st[r].sdspls || !st[r].rcounts || !st[r].rdspls) goto done;
+        comm_data(st[r].send.v, st[r].send.n, r, P, st[r].scounts, st[r].sdspls, &st[r].stotal);
+        comm_data(st[r].recv.v, st[r].recv.n, r, P, st[r].rcounts, st[r].rdspls, &st[r].rtotal);
+        /* uninitialised in the reference (communication_data.cpp:86) */
+        st[r].sbuf = (unsigned char *)malloc((size_t)(st[r].stotal > 0 ? st[r].stotal : 1) * es);
+        st[r].rbuf = (unsigned char *)malloc((size_t)(st[r].rtotal > 0 ? st[r].rtotal : 1) * es);
+        if (!st[r].sbuf || !st[r].rbuf) goto done;
+    }
+    /* pack: copy_to_buffer (transform.cpp:68) */
+    for (int r = 0; r < P; ++r) {
+        for (int64_t i = 0; i < st[r].send.n; ++i) {
+            const orc_msg *m = &st[r].send.v[i];
+            if (m->offset < 0) continue;
+            pack_msg(dtype, es, &inits[m->tag], m, st[r].sbuf + (size_t)m->offset * es);
+        }
+    }
+    /* exchange: Isend(send.dspls[d], send.counts[d]) -> Irecv(recv.dspls[s], recv.counts[s]) */
+    for (int s = 0; s < P; ++s) {
+        for (int d = 0; d < P; ++d) {
+            int64_t cnt = st[s].scounts[d];
+            if (!cnt) continue;
+            if (st[d].rcounts[s] != cnt) goto done;
+            memcpy(st[d].rbuf + (size_t)st[d].rdspls[s] * es, st[s].sbuf + (size_t)st[s].sdspls[d] * es,
+                   (size_t)cnt * es);
+        }
+    }
+    /* local copies: copy_local_blocks (transform.cpp:94, communication_data.cpp:224-252),
+     * from[i] paired with to[i] in sort order */
+    for (int r = 0; r < P; ++r) {
+        int64_t ia = 0, ib = 0;
+        for (;;) {
+            while (ia < st[r].send.n && st[r].send.v[ia].rank != r) ++ia;
+            while (ib < st[r].recv.n && st[r].recv.v[ib].rank != r) ++ib;
+            if (ia >= st[r].send.n || ib >= st[r].recv.n) break;
+            const orc_msg *a = &st[r].send.v[ia], *b = &st[r].recv.v[ib];
+            if (a->r0 != b->r0 || a->r1 != b->r1 || a->c0 != b->c0 || a->c1 != b->c1 || a->tag != b->tag)
+                goto done;
+            int k = a->tag;
+            const unsigned char *al = alphas ? alphas + (size_t)k * es : 0;
+            const unsigned char *be = betas ? betas + (size_t)k * es : 0;
+            int plain = is_plain_copy(dtype, al, be);
+            const unsigned char *src = msg_ptr(&inits[k], a);
+            unsigned char *dst = msg_ptr(&fins[k], b);
+            int64_t nr = a->r1 - a->r0, nc = a->c1 - a->c0;
+            if (!a->transposed)
+                copy2d(dtype, es, nr, nc, src, a->stride, dst, b->stride, plain, al, be);
+            else
+                copy_transpose(dtype, es, nc, nr, src, a->stride, dst, b->stride, a->conjugated, plain,
+                               al, be);
+            ++ia;
+            ++ib;
+        }
+        /* both lists hold the same local pieces */
+        while (ia < st[r].send.n && st[r].send.v[ia].rank != r) ++ia;
+        while (ib < st[r].recv.n && st[r].recv.v[ib].rank != r) ++ib;
+        if (ia < st[r].send.n || ib < st[r].recv.n) goto done;
+    }
+    /* unpack: copy_from_buffer (transform.cpp:109, communication_data.cpp:157-173) */
+    for (int r = 0; r < P; ++r) {
+        for (int64_t i = 0; i < st[r].recv.n; ++i) {
+            const orc_msg *m = &st[r].recv.v[i];
+            if (m->offset < 0) continue;
+            int k = m->tag;
+            const unsigned char *al = alphas ? alphas + (size_t)k * es : 0;
+            const unsigned char *be = betas ? betas + (size_t)k * es : 0;
+            int plain = is_plain_copy(dtype, al, be);
+            int64_t nr = m->r1 - m->r0, nc = m->c1 - m->c0;
+            copy2d(dtype, es, nr, nc, st[r].rbuf + (size_t)m->offset * es, nr, msg_ptr(&fins[k], m),
+                   m->stride, plain, al, be);
+        }
+    }
+    rc = 0;
+done:
+    for (int r = 0; r < P; ++r) {
+        free(st[r].send.v);
+        free(st[r].recv.v);
+        free(st[r].scounts);
+        free(st[r].sdspls);
+        free(st[r].rcounts);
+        free(st[r].rdspls);
+        free(st[r].sbuf);
+        free(st[r].rbuf);
+    }
+    free(st);
+    return rc;
+}
+
+int orc_transform(const orc_layout *init, const orc_layout *fin, int dtype, char trans,
+                  const void *alpha, const void *beta) {
+    char t = trans ? trans : 'N';
+    return run_pipeline(1, init, fin, dtype, &t, (const unsigned char *)alpha,
+                        (const unsigned char *)beta);
+}
+
+int orc_transform_batched(int n_pairs, const orc_layout *inits, const orc_layout *fins,
+                          int dtype, const char *trans, const void *alphas,
+                          const void *betas) {
+    if (n_pairs <= 0) return 0;
+    char *t = (char *)malloc((size_t)n_pairs);
+    if (!t) return -1;
+    for (int k = 0; k < n_pairs; ++k) t[k] = trans ? trans[k] : 'N';
+    int rc = run_pipeline(n_pairs, inits, fins, dtype, t, (const unsigned char *)alphas,
+                          (const unsigned char *)betas);
+    free(t);
+    return rc;
+}
+
+/* ------------------------------------------------------ layout factory */
+
+/* scalapack_layout.cpp:152-177 */
+int orc_line_split(int begin, int end, int blk_len, int32_t *out, int max_out) {
+    int len = end - begin;
+    int rem = blk_len - begin % blk_len;
+    int n = 0;
+#define PUSH(v)                          \
+    do {                                 \
+        if (n >= max_out) return -1;     \
+        out[n++] = (v);                  \
+    } while (0)
+    PUSH(0);
+    if (rem >= len) {
+        PUSH(len);
+        return n;
+    }
+    if (rem != 0) PUSH(rem);
+    int num_blocks = (len - rem) / blk_len;
+    for (int i = 0; i < num_blocks; ++i) PUSH(out[n - 1] + blk_len);
+    if (out[n - 1] != len) PUSH(len);
+#undef PUSH
+    return n;
+}
+
+/* scalapack_layout.cpp:40-57 (rank_from_grid); ordering 1 = column_major */
+static int rank_from_grid(int row, int col, int prow, int pcol, int ordering) {
+    if (row < 0 || row >= prow || col < 0 || col >= pcol) return -1;
+    if (ordering == 1) return col * prow + row;
+    return row * pcol + col;
+}
+
+/* scalapack_layout.cpp:178-278 */
+int orc_scalapack_grid(int lld, int m, int n, int ia, int ja, int subm, int subn, int mb,
+                       int nb, int prow, int pcol, int ordering, int src_row, int src_col,
+                       int rank, int32_t *rows_split, int *n_rows_split, int32_t *cols_split,
+                       int *n_cols_split, int32_t *owners, int32_t *blk_row, int32_t *blk_col,
+                       int64_t *offsets, int max_blocks) {
+    (void)m;
+    (void)n;
+    if (ia < 1 || ja < 1) return -1;
+    ia--;
+    ja--;
+    int nrs = orc_line_split(ia, ia + subm, mb, rows_split, *n_rows_split);
+    int ncs = orc_line_split(ja, ja + subn, nb, cols_split, *n_cols_split);
+    if (nrs < 0 || ncs < 0) return -1;
+    *n_rows_split = nrs;
+    *n_cols_split = ncs;
+    int gr = nrs - 1, gc = ncs - 1;
+    int border_row = ia / mb, border_col = ja / nb;
+    int sub_src_row = (border_row % prow + src_row) % prow;
+    int sub_src_col = (border_col % pcol + src_col) % pcol;
+    int nblk = 0;
+    for (int j = 0; j < gc; ++j) {
+        int rank_col = (j % pcol + sub_src_col) % pcol;
+        for (int i = 0; i < gr; ++i) {
+            int rank_row = (i % prow + sub_src_row) % prow;
+            int o = rank_from_grid(rank_row, rank_col, prow, pcol, ordering);
+            if (o < 0) return -1;
+            owners[(int64_t)i * gc + j] = o;
+            if (o == rank) {
+                int blk_loc_row = (border_row + i) / prow;
+                int blk_loc_col = (border_col + j) / pcol;
+                int sub_loc_row = ia + rows_split[i] - (border_row + i) * mb;
+                int sub_loc_col = ja + cols_split[j] - (border_col + j) * nb;
+                int64_t off = (int64_t)blk_loc_row * mb + sub_loc_row +
+                              (int64_t)lld * ((int64_t)blk_loc_col * nb + sub_loc_col);
+                if (nblk >= max_blocks) return -1;
+                blk_row[nblk] = i;
+                blk_col[nblk] = j;
+                offsets[nblk] = off;
+                nblk++;
+            }
+        }
+    }
+    return nblk;
+}
+
+/* ------------------------------------------------------- seeded values */
+
+uint64_t orc_splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+void orc_value(int dtype, uint64_t seed, int64_t i, int64_t j, int64_t n_cols, void *out) {
+    uint64_t key = (uint64_t)i * (uint64_t)n_cols + (uint64_t)j;
+    switch (dtype) {
+    case ORC_F32: {
+        uint64_t x = orc_splitmix64(seed ^ key);
+        float v = (float)(x >> 40) * 0x1p-23f - 1.0f;
+        memcpy(out, &v, 4);
+        break;
+    }
+    case ORC_F64: {
+        uint64_t x = orc_splitmix64(seed ^ key);
+        double v = (double)(x >> 11) * 0x1p-52 - 1.0;
+        memcpy(out, &v, 8);
+        break;
+    }
+    case ORC_C64: {
+        uint64_t x = orc_splitmix64(seed ^ key);
+        float v[2] = {(float)(x >> 40) * 0x1p-23f - 1.0f,
+                      (float)((x >> 16) & 0xffffffULL) * 0x1p-23f - 1.0f};
+        memcpy(out, v, 8);
+        break;
+    }
+    case ORC_C128: {
+        uint64_t x0 = orc_splitmix64(seed ^ (2 * key));
+        uint64_t x1 = orc_splitmix64(seed ^ (2 * key + 1));
+        double v[2] = {(double)(x0 >> 11) * 0x1p-52 - 1.0, (double)(x1 >> 11) * 0x1p-52 - 1.0};
+        memcpy(out, v, 16);
+        break;
+    }
+    }
+}
+
+int orc_fill_layout(const orc_layout *l, int dtype, uint64_t seed) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    int64_t n_cols = l->cols_split[l->n_cols_split - 1];
+    for (int b = 0; b < l->n_blocks; ++b) {
+        const orc_block *blk = &l->blocks[b];
+        int R0 = l->rows_split[blk->row], R1 = l->rows_split[blk->row + 1];
+        int C0 = l->cols_split[blk->col], C1 = l->cols_split[blk->col + 1];
+        for (int j = C0; j < C1; ++j)
+            for (int i = R0; i < R1; ++i)
+                orc_value(dtype, seed, i, j, n_cols,
+                          (unsigned char *)blk->data + (size_t)((j - C0) * blk->stride + (i - R0)) * es);
+    }
+    return 0;
+}
+
+int64_t orc_check_layout(const orc_layout *fin, int dtype, uint64_t seed, char trans,
+                         int64_t a_n_cols) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    int t = (trans == 'T' || trans == 'C');
+    int64_t bad = 0;
+    unsigned char want[16];
+    for (int b = 0; b < fin->n_blocks; ++b) {
+        const orc_block *blk = &fin->blocks[b];
+        int R0 = fin->rows_split[blk->row], R1 = fin->rows_split[blk->row + 1];
+        int C0 = fin->cols_split[blk->col], C1 = fin->cols_split[blk->col + 1];
+        for (int j = C0; j < C1; ++j) {
+            for (int i = R0; i < R1; ++i) {
+                if (t)
+                    orc_value(dtype, seed, j, i, a_n_cols, want);
+                else
+                    orc_value(dtype, seed, i, j, a_n_cols, want);
+                if (trans == 'C' && is_complex(dtype)) conj_el(dtype, want);
+                const unsigned char *got =
+                    (const unsigned char *)blk->data + (size_t)((j - C0) * blk->stride + (i - R0)) * es;
+                if (memcmp(got, want, es) != 0) bad++;
+            }
+        }
+    }
+    return bad;
+}

@@ -1,0 +1,233 @@
+"""GPU parity: the HIP transform (fused gather and plain staged form),
+called through the C ABI, against the CPU oracle on the same bytes.
+
+Arenas are filled with random bytes (NaN payloads, -0.0, denormals
+included) so bit-exactness is checked on arbitrary patterns; every byte of
+the destination arena is compared, so ld padding and gaps between blocks
+must stay untouched."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import grid2grid_amd as g2g
+from grid2grid_amd.layout import scalapack_layout, arbitrary_layout
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DTYPES = [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64, g2g.COMPLEX128]
+
+
+def _dev(host):
+    return torch.from_numpy(host).to("cuda")
+
+
+def _rand_arena(layout, dtype, rng):
+    return rng.integers(0, 256, size=layout.arena_elems * g2g.DTYPE_SIZE[dtype], dtype=np.uint8)
+
+
+def _layout_pairs():
+    rng = np.random.default_rng(101)
+    return {
+        "bc_small": (scalapack_layout(37, 29, 4, 3, 2, 2, "C"), scalapack_layout(37, 29, 5, 7, 2, 2, "C")),
+        "bc_grid_change": (scalapack_layout(100, 77, 8, 8, 2, 3, "R"), scalapack_layout(100, 77, 3, 5, 3, 1, "C")),
+        "bc_cfg1_shape": (scalapack_layout(256, 256, 32, 32, 2, 2, "C", lld="rows"),
+                          scalapack_layout(256, 256, 64, 128, 2, 2, "C", lld="rows")),
+        "arbitrary": (arbitrary_layout(rng, 300, 260, 7, 6, 5, lo=10, hi=80, pad=(0, 9), gap=(0, 33)),
+                      arbitrary_layout(rng, 300, 260, 5, 8, 6, lo=10, hi=80, pad=(0, 9), gap=(0, 33))),
+        "single_rank": (scalapack_layout(64, 48, 64, 48, 1, 1), scalapack_layout(64, 48, 16, 16, 1, 1)),
+        "tall_thin": (scalapack_layout(1000, 3, 17, 1, 4, 1), scalapack_layout(1000, 3, 300, 2, 2, 2)),
+    }
+
+
+def _transposed_pairs():
+    rng = np.random.default_rng(202)
+    return {
+        "bcT": (scalapack_layout(70, 93, 8, 5, 2, 2, "C"), scalapack_layout(93, 70, 6, 9, 2, 2, "C")),
+        "arbT": (arbitrary_layout(rng, 130, 210, 4, 5, 4, lo=10, hi=80, pad=(0, 6), gap=(0, 17)),
+                 arbitrary_layout(rng, 210, 130, 6, 3, 3, lo=10, hi=80, pad=(0, 6), gap=(0, 17))),
+        "bcT_big_tiles": (scalapack_layout(160, 96, 64, 48, 2, 1, "C"), scalapack_layout(96, 160, 40, 64, 1, 2, "C")),
+    }
+
+
+def _expected(init, ia, fin, fa, dtype, trans="N", alpha=None, beta=None):
+    out = fa.copy()
+    O.transform(init, ia.copy(), fin, out, dtype, trans, alpha, beta)
+    return out
+
+
+def _run(init, ia, fin, fa, dtype, trans="N", alpha=None, beta=None, staged=False):
+    ti, tf = _dev(ia), _dev(fa)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+    if staged:
+        plan.transform_staged(alpha, beta)
+    else:
+        plan.transform(alpha, beta)
+    torch.cuda.synchronize()
+    got = tf.cpu().numpy()
+    # the source is never modified
+    np.testing.assert_array_equal(ti.cpu().numpy(), ia)
+    plan.close()
+    return got
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["fused", "staged"])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", list(_layout_pairs()))
+def test_transform_bit_exact(name, dtype, staged):
+    init, fin = _layout_pairs()[name]
+    rng = np.random.default_rng(hash((name, dtype)) % 2**32)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    print(f"case {name} dtype {dtype} staged {staged}", flush=True)
+    want = _expected(init, ia, fin, fa, dtype)
+    got = _run(init, ia, fin, fa, dtype, staged=staged)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["fused", "staged"])
+@pytest.mark.parametrize("trans", ["T", "C"])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", list(_transposed_pairs()))
+def test_transform_transposed_bit_exact(name, dtype, trans, staged):
+    init, fin = _transposed_pairs()[name]
+    rng = np.random.default_rng(hash((name, dtype, trans)) % 2**32)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    print(f"case {name} dtype {dtype} trans {trans} staged {staged}", flush=True)
+    want = _expected(init, ia, fin, fa, dtype, trans)
+    got = _run(init, ia, fin, fa, dtype, trans, staged=staged)
+    np.testing.assert_array_equal(got, want)
+
+
+SCALARS = {
+    g2g.FLOAT32: (np.float32(0.75), np.float32(-1.25)),
+    g2g.FLOAT64: (0.3, 1.7),
+    g2g.COMPLEX64: (np.complex64(0.5 - 1.5j), np.complex64(-0.25 + 2.0j)),
+    g2g.COMPLEX128: (0.7 + 0.1j, -1.1 + 0.9j),
+}
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["fused", "staged"])
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_scaled_bit_exact(dtype, trans, staged):
+    """beta*dst + alpha*op(src) on finite seeded values: identical bits to the
+    oracle (no FMA contraction on the GPU)."""
+    pairs = _layout_pairs() if trans == "N" else _transposed_pairs()
+    alpha, beta = SCALARS[dtype]
+    es = g2g.DTYPE_SIZE[dtype]
+    for name in list(pairs)[:3]:
+        init, fin = pairs[name]
+        ia = np.zeros(init.arena_elems * es, dtype=np.uint8)
+        fa = np.zeros(fin.arena_elems * es, dtype=np.uint8)
+        O.fill(init, ia, dtype, 7)
+        O.fill(fin, fa, dtype, 8)
+        want = _expected(init, ia, fin, fa, dtype, trans, alpha, beta)
+        got = _run(init, ia, fin, fa, dtype, trans, alpha, beta, staged=staged)
+        np.testing.assert_array_equal(got, want, err_msg=name)
+
+
+@pytest.mark.parametrize("name", ["bc_small", "bc_grid_change", "arbitrary"])
+def test_wire_order_matches_oracle(name):
+    """Counts per (source, destination) and every piece's offset in the send
+    and receive buffers equal the reference's communication_data (via the
+    oracle, itself pinned to the reference)."""
+    init, fin = _layout_pairs()[name]
+    dtype = g2g.FLOAT64
+    rng = np.random.default_rng(5)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    ti, tf = _dev(ia), _dev(fa)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+    counts = plan.counts()
+    w = plan.wire()
+    P = max(init.n_ranks, fin.n_ranks)
+    for r in range(P):
+        msgs, c_send, _ = O.plan_rank(init, ia, fin, fa, dtype, "N", r, 0)
+        np.testing.assert_array_equal(counts[r], c_send)
+        want = {(m.r0, m.c0): m.offset for m in msgs if m.offset >= 0}
+        sel = (w["src"] == r) & (w["dst"] != r)
+        got = {(int(a), int(b)): int(o) for a, b, o in zip(w["row0"][sel], w["col0"][sel], w["send_off"][sel])}
+        assert got == want
+        msgs, c_recv, _ = O.plan_rank(init, ia, fin, fa, dtype, "N", r, 1)
+        np.testing.assert_array_equal(counts[:, r], c_recv)
+        want = {(m.r0, m.c0): m.offset for m in msgs if m.offset >= 0}
+        sel = (w["dst"] == r) & (w["src"] != r)
+        got = {(int(a), int(b)): int(o) for a, b, o in zip(w["row0"][sel], w["col0"][sel], w["recv_off"][sel])}
+        assert got == want
+    plan.close()
+
+
+_hip = None
+
+
+def _hip_memcpy_d2h(ptr, nbytes):
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.zeros(nbytes, dtype=np.uint8)
+    if nbytes:
+        assert _hip.hipMemcpy(out.ctypes.data, ptr, nbytes, 2) == 0
+    return out
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_staged_send_buffers_match_reference_packing(trans):
+    """After the plain form ran, each virtual rank's packed send buffer holds
+    the same bytes as the reference's copy_to_buffer (via the oracle)."""
+    name = "bc_grid_change" if trans == "N" else "bcT"
+    init, fin = (_layout_pairs() if trans == "N" else _transposed_pairs())[name]
+    dtype = g2g.COMPLEX64
+    rng = np.random.default_rng(9)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    ti, tf = _dev(ia), _dev(fa)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+    plan.transform_staged()
+    torch.cuda.synchronize()
+    for r in range(max(init.n_ranks, fin.n_ranks)):
+        ptr, n = plan.send_buffer(r)
+        got = _hip_memcpy_d2h(ptr, n * g2g.DTYPE_SIZE[dtype])
+        want = O.pack_rank(init, ia, fin, fa, dtype, trans, r)
+        np.testing.assert_array_equal(got, want)
+    plan.close()
+
+
+def test_repeat_and_reuse_plan():
+    """A plan is reusable; running it twice (and staged after fused) gives
+    the same bytes."""
+    init, fin = _layout_pairs()["arbitrary"]
+    dtype = g2g.FLOAT32
+    rng = np.random.default_rng(1)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    want = _expected(init, ia, fin, fa, dtype)
+    ti, tf = _dev(ia), _dev(fa)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+    for k in range(3):
+        tf.copy_(_dev(fa))
+        (plan.transform_staged if k == 1 else plan.transform)()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want)
+    plan.close()
+
+
+@pytest.mark.parametrize("tile_bytes,persistent", [(4096, 0), (4096, 1), (65536, 1), (1 << 20, 0)])
+def test_work_item_sizes(tile_bytes, persistent):
+    """Results do not depend on the work-item size or launch mode."""
+    init, fin = _layout_pairs()["arbitrary"]
+    dtype = g2g.COMPLEX128
+    rng = np.random.default_rng(77)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    want = _expected(init, ia, fin, fa, dtype)
+    g2g.set_tuning(tile_bytes, persistent)
+    try:
+        got = _run(init, ia, fin, fa, dtype)
+        got_t = None
+        initT, finT = _transposed_pairs()["arbT"]
+        iaT, faT = _rand_arena(initT, dtype, rng), _rand_arena(finT, dtype, rng)
+        wantT = _expected(initT, iaT, finT, faT, dtype, "C")
+        got_t = _run(initT, iaT, finT, faT, dtype, "C")
+    finally:
+        g2g.set_tuning(0, -1)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got_t, wantT)

@@ -7,10 +7,12 @@ the tests and the benchmark; it has no CPU fallback.
 from ._lib import (FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, DTYPE_SIZE, DTYPE_NAMES, G2GError,
                    LIB_PATH, load)
 from .layout import Layout, scalapack_layout, arbitrary_layout, line_split, random_splits
-from .api import Plan, Batch, set_tuning, fill_layout, verify_layout
+from .api import (Plan, Batch, Server, SERVE_BATCHED, SERVE_PLAIN, set_tuning, fill_layout,
+                  verify_layout)
 
 __all__ = [
     "FLOAT32", "FLOAT64", "COMPLEX64", "COMPLEX128", "DTYPE_SIZE", "DTYPE_NAMES", "G2GError",
     "LIB_PATH", "load", "Layout", "scalapack_layout", "arbitrary_layout", "line_split",
-    "random_splits", "Plan", "Batch", "set_tuning", "fill_layout", "verify_layout",
+    "random_splits", "Plan", "Batch", "Server", "SERVE_BATCHED", "SERVE_PLAIN", "set_tuning",
+    "fill_layout", "verify_layout",
 ]

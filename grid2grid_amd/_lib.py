@@ -69,6 +69,7 @@ _p32 = ctypes.POINTER(ctypes.c_int32)
 PROTOTYPES = {
     "g2g_last_error": (ctypes.c_char_p, []),
     "g2g_version": (_i, []),
+    "g2g_synchronize": (_i, [_vp]),
     "g2g_plan_create": (_i, [ctypes.POINTER(G2GLayout), ctypes.POINTER(G2GLayout), _i, ctypes.c_char,
                              _vp, ctypes.POINTER(_vp)]),
     "g2g_plan_destroy": (_i, [_vp]),
@@ -85,6 +86,13 @@ PROTOTYPES = {
     "g2g_batch_bytes": (_i, [_vp, _p64]),
     "g2g_batch_destroy": (_i, [_vp]),
     "g2g_set_tuning": (_i, [_i64, _i]),
+    "g2g_server_create": (_i, [_i64, _i, _i, _i, ctypes.POINTER(_vp)]),
+    "g2g_server_submit": (_i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
+    "g2g_server_poll": (_i, [_vp, _i64, _p32, ctypes.POINTER(ctypes.c_double)]),
+    "g2g_server_wait": (_i, [_vp, _i64, ctypes.POINTER(ctypes.c_double)]),
+    "g2g_server_drain": (_i, [_vp]),
+    "g2g_server_get_stats": (_i, [_vp, ctypes.POINTER(G2GServerStats)]),
+    "g2g_server_destroy": (_i, [_vp]),
     "g2g_fill_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, _vp]),
     "g2g_verify_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, ctypes.c_char, _i64, _vp,
                                _p64]),
@@ -100,6 +108,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m grid2grid_amd.build`")
+    # torch ships its own libamdhip64.so.7; when torch is present it must be
+    # loaded first so that the library binds to that same HIP runtime (one
+    # set of streams and queues per process) instead of a second copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)

@@ -157,6 +157,56 @@ class Batch:
             pass
 
 
+SERVE_BATCHED, SERVE_PLAIN = 0, 1
+
+
+class Server:
+    """Request loop with continuous batching under an HBM budget (g2g_server_*)."""
+
+    def __init__(self, budget_bytes, n_streams=2, max_batch=256, mode=SERVE_BATCHED):
+        h = ctypes.c_void_p()
+        check(_lib.load().g2g_server_create(int(budget_bytes), n_streams, max_batch, mode, ctypes.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, plan, alpha=None, beta=None):
+        a = _scalar_bytes(plan.dtype, alpha)
+        b = _scalar_bytes(plan.dtype, beta)
+        t = ctypes.c_int64()
+        check(_lib.load().g2g_server_submit(self.handle, plan.handle, a, b, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        done = ctypes.c_int32()
+        lat = ctypes.c_double()
+        check(_lib.load().g2g_server_poll(self.handle, ticket, ctypes.byref(done), ctypes.byref(lat)))
+        return (lat.value if done.value else None)
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        check(_lib.load().g2g_server_wait(self.handle, ticket, ctypes.byref(lat)))
+        return lat.value
+
+    def drain(self):
+        check(_lib.load().g2g_server_drain(self.handle))
+
+    def stats(self):
+        s = _lib.G2GServerStats()
+        check(_lib.load().g2g_server_get_stats(self.handle, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().g2g_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def set_tuning(tile_bytes=0, persistent=-1):
     check(_lib.load().g2g_set_tuning(int(tile_bytes), int(persistent)))
 

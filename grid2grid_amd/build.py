@@ -18,6 +18,8 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-O3",
     "--offload-arch=gfx950",
+    # code object v5: loadable by the HIP runtime bundled with PyTorch too
+    "-mcode-object-version=5",
     "-fPIC",
     "-shared",
     # no FMA contraction anywhere: the scaled path must round like the reference
@@ -56,6 +58,29 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_cpp_tests(verbose=False):
+    """C++ programs under tests/cpp written against the header-only facade
+    (include/grid2grid), linked to the library; built into tests/cpp/bin."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    built = []
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        out = os.path.join(out_dir, f[:-4])
+        cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+               os.path.join(src_dir, f), "-L" + HERE, "-lgrid2grid_amd",
+               "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../grid2grid_amd", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {f} failed:\n{r.stderr}")
+        built.append(out)
+    return built
+
+
 def build_oracle(verbose=False):
     """Compile the C oracle and, when the reference tree is present, the
     reference harness (oracle/_ref). Test infrastructure only."""
@@ -69,6 +94,7 @@ def build_oracle(verbose=False):
 
 def main():
     build_library(verbose=True)
+    build_cpp_tests(verbose=True)
     build_oracle(verbose=True)
     print("built", LIB)
 

@@ -278,10 +278,9 @@ int grid_limit() { return device_cus() * 8; }
 
 // bytes -> inclusive scan -> first piece per tile for a stage table whose
 // pieces are already on the device.
-int finalize_table(stage_table &t, const request *reqtab, const kreqs &kr, uint64_t total_hint,
-                   hipStream_t s) {
+int finalize_table(stage_table &t, const kreqs &kr, uint64_t total_hint, hipStream_t s) {
     G2G_TRY(t.pend.alloc((size_t)t.n));
-    G2G_HIP(launch_piece_bytes(t.pieces.p, t.n, reqtab, kr, t.pend.p, s));
+    G2G_HIP(launch_piece_bytes(t.pieces.p, t.n, kr, t.pend.p, s));
     dbuf<uint64_t> scratch;
     G2G_TRY(scratch.alloc(scan_scratch_elems(t.n)));
     G2G_HIP(inclusive_scan_u64(t.pend.p, t.n, scratch.p, s));
@@ -437,24 +436,32 @@ int ensure_staged(g2g_plan p, hipStream_t s) {
     std::memset(&kr, 0, sizeof kr);
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, false, false);
     kr.r[1] = kr.r[0];
-    G2G_TRY(finalize_table(p->sa, nullptr, kr, (uint64_t)p->m * p->n * es, s));
-    G2G_TRY(finalize_table(p->sb, nullptr, kr, (uint64_t)p->remote_elems * es, s));
-    G2G_TRY(finalize_table(p->sc, nullptr, kr, (uint64_t)p->remote_elems * es, s));
+    G2G_TRY(finalize_table(p->sa, kr, (uint64_t)p->m * p->n * es, s));
+    G2G_TRY(finalize_table(p->sb, kr, (uint64_t)p->remote_elems * es, s));
+    G2G_TRY(finalize_table(p->sc, kr, (uint64_t)p->remote_elems * es, s));
     p->staged_ready = true;
     return G2G_OK;
+}
+
+work_desc make_desc(const stage_table &t, const kreqs &kr, uint32_t tile_base) {
+    work_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.pieces = t.pieces.p;
+    d.pend = t.pend.p;
+    d.first_piece = t.first.p;
+    d.tile_bytes = t.tile_bytes;
+    d.total_bytes = t.total;
+    d.n_tiles = t.n_tiles;
+    d.tile_base = tile_base;
+    d.kr = kr;
+    return d;
 }
 
 int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
     gather_args a;
     std::memset(&a, 0, sizeof a);
-    a.pieces = t.pieces.p;
-    a.pend = t.pend.p;
-    a.first_piece = t.first.p;
-    a.reqtab = nullptr;
-    a.kr = kr;
-    a.tile_bytes = t.tile_bytes;
-    a.total_bytes = t.total;
-    a.n_tiles = t.n_tiles;
+    a.one = make_desc(t, kr, 0);
+    a.total_tiles = t.n_tiles;
     a.queue = p->queue.p;
     G2G_HIP(launch_gather(a, persistent_mode(t.n_tiles), grid_limit(), s));
     return G2G_OK;
@@ -467,6 +474,11 @@ extern "C" {
 const char *g2g_last_error(void) { return t_last_error.c_str(); }
 
 int g2g_version(void) { return G2G_ABI_VERSION; }
+
+int g2g_synchronize(void *stream) {
+    G2G_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return G2G_OK;
+}
 
 int g2g_set_tuning(int64_t tile_bytes, int persistent) {
     if (tile_bytes != 0 && (tile_bytes < 4096 || tile_bytes > (1ll << 26)))
@@ -659,12 +671,13 @@ int g2g_plan_send_buffer(g2g_plan p, int rank, const void **data, int64_t *elems
 
 struct g2g_batch_s {
     std::vector<g2g_plan> plans;
-    stage_table t;
-    dbuf<request> reqs;
+    dbuf<work_desc> descs;
+    dbuf<uint32_t> owner;
     dbuf<uint32_t> queue;
+    uint32_t total_tiles = 0;
+    uint64_t total_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t graph_stream = nullptr;
     ~g2g_batch_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -676,19 +689,54 @@ namespace {
 int batch_launch(g2g_batch b, hipStream_t s) {
     gather_args a;
     std::memset(&a, 0, sizeof a);
-    a.pieces = b->t.pieces.p;
-    a.pend = b->t.pend.p;
-    a.first_piece = b->t.first.p;
-    a.reqtab = b->reqs.p;
-    a.tile_bytes = b->t.tile_bytes;
-    a.total_bytes = b->t.total;
-    a.n_tiles = b->t.n_tiles;
+    a.descs = b->descs.p;
+    a.tile_owner = b->owner.p;
+    a.total_tiles = b->total_tiles;
     a.queue = b->queue.p;
     G2G_HIP(launch_gather(a, true, grid_limit(), s));
     return G2G_OK;
 }
 
 }  // namespace
+
+namespace g2g {
+
+int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas, const void *const *betas,
+                   std::vector<work_desc> &out, uint32_t &total_tiles, uint64_t &total_bytes) {
+    out.clear();
+    uint64_t tiles = 0, bytes = 0;
+    for (int k = 0; k < n; ++k) {
+        if (!plans[k]) return fail(G2G_ERR_INVALID, "plan %d is NULL", k);
+        const void *a = alphas ? alphas[k] : nullptr, *be = betas ? betas[k] : nullptr;
+        if ((a == nullptr) != (be == nullptr)) return fail(G2G_ERR_INVALID, "plan %d: alpha/beta mismatch", k);
+        kreqs kr;
+        std::memset(&kr, 0, sizeof kr);
+        kr.r[0] = make_request(plans[k]->dtype, plans[k]->trans, a, be, true, true);
+        kr.r[1] = kr.r[0];
+        out.push_back(make_desc(plans[k]->fused, kr, (uint32_t)tiles));
+        tiles += plans[k]->fused.n_tiles;
+        bytes += plans[k]->fused.total;
+        if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
+    }
+    total_tiles = (uint32_t)tiles;
+    total_bytes = bytes;
+    return G2G_OK;
+}
+
+int internal_grid_limit() { return grid_limit(); }
+
+int internal_fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_last_error = buf;
+    return code;
+}
+
+}  // namespace g2g
+
 
 extern "C" {
 
@@ -703,29 +751,14 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
         g2g_batch &b;
         ~guard() { delete b; }
     } g{b};
-    std::vector<request> rq((size_t)std::max(n, 1));
-    int64_t np = 0;
-    for (int k = 0; k < n; ++k) {
-        if (!plans[k]) return fail(G2G_ERR_INVALID, "plan %d is NULL", k);
-        const void *a = alphas ? alphas[k] : nullptr, *be = betas ? betas[k] : nullptr;
-        if ((a == nullptr) != (be == nullptr)) return fail(G2G_ERR_INVALID, "plan %d: alpha/beta mismatch", k);
-        rq[k] = make_request(plans[k]->dtype, plans[k]->trans, a, be, true, true);
-        np += plans[k]->fused.n;
-        b->plans.push_back(plans[k]);
-    }
-    if (np > 0xffffffffll) return fail(G2G_ERR_UNSUPPORTED, "batch has too many pieces");
-    b->t.n = np;
-    G2G_TRY(b->t.pieces.alloc((size_t)np));
-    G2G_TRY(b->reqs.upload(rq.data(), rq.size(), s));
+    std::vector<work_desc> descs;
+    G2G_TRY(internal_descs(n, plans, alphas, betas, descs, b->total_tiles, b->total_bytes));
+    b->plans.assign(plans, plans + n);
+    G2G_TRY(b->descs.upload(descs.data(), descs.size(), s));
+    G2G_TRY(b->owner.alloc(b->total_tiles));
     G2G_TRY(b->queue.alloc(1));
-    int64_t off = 0;
-    for (int k = 0; k < n; ++k) {
-        G2G_HIP(launch_concat_pieces(plans[k]->fused.pieces.p, plans[k]->fused.n, (uint32_t)k, b->t.pieces.p + off, s));
-        off += plans[k]->fused.n;
-    }
-    kreqs kr;
-    std::memset(&kr, 0, sizeof kr);
-    G2G_TRY(finalize_table(b->t, b->reqs.p, kr, 0, s));
+    G2G_HIP(launch_tile_owner(b->descs.p, n, b->owner.p, s));
+    G2G_HIP(hipStreamSynchronize(s));
     *out = b;
     b = nullptr;
     return G2G_OK;
@@ -759,7 +792,7 @@ int g2g_batch_execute_graph(g2g_batch b, void *stream) {
 
 int g2g_batch_bytes(g2g_batch b, int64_t *bytes) {
     if (!b || !bytes) return fail(G2G_ERR_INVALID, "NULL argument");
-    *bytes = (int64_t)b->t.total;
+    *bytes = (int64_t)b->total_bytes;
     return G2G_OK;
 }
 

@@ -5,6 +5,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+struct g2g_plan_s;
 
 namespace g2g {
 
@@ -78,9 +81,9 @@ hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
                               piece *pieces, uint64_t *bytes, uint32_t *ranks,
                               hipStream_t stream);
 
-// bytes[p] = rows*cols*esize of the piece's request (request table or kreqs).
-hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const request *reqtab,
-                              const kreqs &kr, uint64_t *bytes, hipStream_t stream);
+// bytes[p] = rows*cols*esize of the piece's request.
+hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const kreqs &kr, uint64_t *bytes,
+                              hipStream_t stream);
 
 // In-place inclusive scan of n uint64 values (scratch: see scan_scratch_elems).
 size_t scan_scratch_elems(int64_t n);
@@ -106,24 +109,46 @@ hipError_t launch_stage_pieces(const piece *pieces, const uint32_t *ranks, const
                                uint32_t esize, piece *stage_a, piece *stage_c,
                                hipStream_t stream);
 
-// Copies n pieces setting their request index.
-hipError_t launch_concat_pieces(const piece *src, int64_t n, uint32_t req, piece *dst,
-                                hipStream_t stream);
-
-// The gather over a finalized piece table.
-struct gather_args {
+// One transform (or one stage of the plain form) as the gather sees it: a
+// finalized piece table, its work items and the requests its pieces use.
+struct work_desc {
     const piece *pieces;
-    const uint64_t *pend;
-    const uint32_t *first_piece;
-    const request *reqtab;  // may be null: kr is used
-    kreqs kr;
+    const uint64_t *pend;         // inclusive byte scan over pieces
+    const uint32_t *first_piece;  // per work item
     uint64_t tile_bytes;
     uint64_t total_bytes;
     uint32_t n_tiles;
-    uint32_t *queue;        // persistent mode work counter (reset by the launcher)
+    uint32_t tile_base;           // first global work index of this desc in a launch
+    kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1]
+};
+
+// A launch: one desc passed by value, or a device table of descs with the
+// owner of every global work index (multi-transform launches).
+struct gather_args {
+    const work_desc *descs;       // device table, or null: `one` is used
+    const uint32_t *tile_owner;   // descs[tile_owner[w]] owns work index w
+    work_desc one;
+    uint32_t total_tiles;
+    uint32_t *queue;              // persistent-mode work counter (reset by the launcher)
 };
 hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
                          hipStream_t stream);
+
+// tile_owner[w] = k for w in [descs[k].tile_base, + n_tiles), k < n.
+hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_owner,
+                             hipStream_t stream);
+
+// ------------------------------------------------ host helpers (g2g_host.cpp)
+
+// Work descriptors of n plans for one multi-transform launch; tile_base is
+// the running prefix of work items. Returns a g2g_status.
+int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas,
+                   const void *const *betas, std::vector<work_desc> &out, uint32_t &total_tiles,
+                   uint64_t &total_bytes);
+// Workgroups of a persistent launch filling the device.
+int internal_grid_limit();
+// Records the thread's last error message and returns code.
+int internal_fail(int code, const char *fmt, ...);
 
 // Test-support kernels.
 struct fill_block {

@@ -262,7 +262,7 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
 // stream (pieces concatenated in table order, each column-major). It moves
 // every column (every 32-column group for transposed pieces) whose first
 // byte lies in that range, so work items partition the columns exactly.
-__device__ __forceinline__ void process_tile(const gather_args &a, uint32_t t,
+__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
                                              unsigned char *smem) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
     const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
@@ -270,7 +270,7 @@ __device__ __forceinline__ void process_tile(const gather_args &a, uint32_t t,
         const uint64_t pstart = p ? a.pend[p - 1] : 0;
         if (pstart >= hi) break;
         const piece pc = a.pieces[p];
-        const request rq = a.reqtab ? a.reqtab[pc.req] : (pc.req ? a.kr.r[1] : a.kr.r[0]);
+        const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
         const uint64_t colb = (uint64_t)pc.rows * rq.esize;
         const uint32_t q = (rq.flags & REQ_TRANS) ? (uint32_t)kTileDim : 1u;
         const uint64_t qb = colb * q;
@@ -282,11 +282,25 @@ __device__ __forceinline__ void process_tile(const gather_args &a, uint32_t t,
     }
 }
 
+// Global work index w -> (transform, local work item) -> columns to move.
+__device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
+                                             unsigned char *smem) {
+    if (!a.descs) {
+        process_tile(a.one, w, smem);
+        return;
+    }
+    const uint32_t k = a.tile_owner[w];
+    const work_desc d = a.descs[k];
+    process_tile(d, w - d.tile_base, smem);
+}
+
+// Direct mode: one workgroup per work index. Persistent mode: a grid sized to
+// the chip pulls work indices from a device counter until all are taken.
 template <bool PERSIST>
 __global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[kTileDim * (kTileDim + 1) * 16];
     if (!PERSIST) {
-        process_tile(a, blockIdx.x, smem);
+        process_work(a, blockIdx.x, smem);
         return;
     }
     __shared__ uint32_t s_t;
@@ -295,23 +309,38 @@ __global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
         __syncthreads();
         const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
-        if (t >= a.n_tiles) break;
-        process_tile(a, t, smem);
+        if (t >= a.total_tiles) break;
+        process_work(a, t, smem);
     }
 }
 
+// Resets the work counter; a kernel (not a memset) so that it can sit in a
+// captured hipGraph.
 __global__ void k_reset_queue(uint32_t *q) { *q = 0; }
 
 hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
                          hipStream_t stream) {
-    if (a.n_tiles == 0) return hipSuccess;
+    if (a.total_tiles == 0) return hipSuccess;
     if (!persistent) {
-        hipLaunchKernelGGL(k_gather<false>, dim3(a.n_tiles), dim3(kThreads), 0, stream, a);
+        hipLaunchKernelGGL(k_gather<false>, dim3(a.total_tiles), dim3(kThreads), 0, stream, a);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
-    const uint32_t grid = std::min<uint32_t>(a.n_tiles, (uint32_t)std::max(1, grid_limit));
+    const uint32_t grid = std::min<uint32_t>(a.total_tiles, (uint32_t)std::max(1, grid_limit));
     hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(kThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+__global__ void k_tile_owner(const work_desc *descs, uint32_t *owner) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t base = descs[k].tile_base, n = descs[k].n_tiles;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) owner[base + t] = k;
+}
+
+hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_owner,
+                             hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_owner, dim3((unsigned)n), dim3(256), 0, stream, descs, tile_owner);
     return hipGetLastError();
 }
 
@@ -368,20 +397,19 @@ hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
     return hipGetLastError();
 }
 
-__global__ void k_piece_bytes(const piece *pieces, int64_t n, const request *reqtab, kreqs kr,
-                              uint64_t *bytes) {
+__global__ void k_piece_bytes(const piece *pieces, int64_t n, kreqs kr, uint64_t *bytes) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const piece pc = pieces[p];
-    const uint32_t es = reqtab ? reqtab[pc.req].esize : (pc.req ? kr.r[1].esize : kr.r[0].esize);
+    const uint32_t es = pc.req ? kr.r[1].esize : kr.r[0].esize;
     bytes[p] = (uint64_t)pc.rows * pc.cols * es;
 }
 
-hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const request *reqtab,
-                              const kreqs &kr, uint64_t *bytes, hipStream_t stream) {
+hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const kreqs &kr, uint64_t *bytes,
+                              hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_piece_bytes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
-                       pieces, n, reqtab, kr, bytes);
+                       pieces, n, kr, bytes);
     return hipGetLastError();
 }
 
@@ -623,22 +651,6 @@ hipError_t launch_stage_pieces(const piece *pieces, const uint32_t *ranks, const
     hipLaunchKernelGGL(k_stage_pieces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pieces,
                        ranks, within, n, n_ranks, sbase, sdspl, rbase, rdspl, send_arena, recv_arena,
                        esize, stage_a, stage_c);
-    return hipGetLastError();
-}
-
-__global__ void k_concat(const piece *src, int64_t n, uint32_t req, piece *dst) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    piece pc = src[p];
-    pc.req = req;
-    dst[p] = pc;
-}
-
-hipError_t launch_concat_pieces(const piece *src, int64_t n, uint32_t req, piece *dst,
-                                hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_concat, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n, req,
-                       dst);
     return hipGetLastError();
 }
 

@@ -100,6 +100,12 @@ const char *g2g_last_error(void);
 /* Library version / build identification. */
 int g2g_version(void);
 
+/* Wait until all work queued on `stream` finished (hipStreamSynchronize).
+ * The reference's transform returns with the data in place
+ * (transform.cpp:117-123, MPI_Waitall); callers that need that guarantee
+ * after an asynchronous call use this. */
+int g2g_synchronize(void *stream);
+
 /* Build a plan for transforming `initial` into `final`, as
  * grid2grid::transform(initial, final, trans, alpha, beta, comm) does
  * (reference transform.cpp:126-164). The planning of the reference --
@@ -182,6 +188,55 @@ int g2g_batch_execute_graph(g2g_batch batch, void *stream);
 int g2g_batch_bytes(g2g_batch batch, int64_t *bytes);
 
 int g2g_batch_destroy(g2g_batch batch);
+
+/* ---------------------------------------------------------------- serving */
+
+/* A request loop on its own host thread: transform requests (plans whose
+ * buffers and interval tables stay resident) are submitted as they arrive,
+ * admitted while the HBM footprint of the requests in flight (2*m*n*sizeof(T)
+ * each: source plus destination) stays within `budget_bytes`, and packed
+ * into persistent launches together with whatever else is waiting
+ * (continuous batching); up to `n_streams` launches overlap on the device.
+ * mode G2G_SERVE_BATCHED: fused gather, up to `max_batch` requests per
+ * launch. mode G2G_SERVE_PLAIN: one request per launch with the plain staged
+ * form, on one stream (the per-request baseline). No counterpart in the
+ * reference, whose transform is a blocking collective call. */
+typedef struct g2g_server_s *g2g_server;
+
+enum { G2G_SERVE_BATCHED = 0, G2G_SERVE_PLAIN = 1 };
+
+typedef struct g2g_server_stats {
+    int64_t submitted;
+    int64_t completed;
+    int64_t rejected;            /* larger than the whole budget */
+    int64_t launches;
+    int64_t bytes;               /* matrix bytes of completed requests */
+    int64_t in_flight_bytes;     /* footprint admitted and not completed */
+    int64_t peak_in_flight_bytes;
+} g2g_server_stats;
+
+int g2g_server_create(int64_t budget_bytes, int n_streams, int max_batch, int mode,
+                      g2g_server *out);
+
+/* Queue plan for execution (alpha/beta as for g2g_transform, copied).
+ * Returns a ticket; G2G_ERR_BUDGET if the request can never fit the budget.
+ * A plan must not be submitted again before its previous ticket completed. */
+int g2g_server_submit(g2g_server s, g2g_plan plan, const void *alpha, const void *beta,
+                      int64_t *ticket);
+
+/* *done = 1 and *latency_s = completion - submission time once finished. */
+int g2g_server_poll(g2g_server s, int64_t ticket, int32_t *done, double *latency_s);
+
+/* Block until the ticket finished. */
+int g2g_server_wait(g2g_server s, int64_t ticket, double *latency_s);
+
+/* Block until every submitted request finished. */
+int g2g_server_drain(g2g_server s);
+
+int g2g_server_get_stats(g2g_server s, g2g_server_stats *stats);
+
+/* Drains, stops the thread and frees the server. */
+int g2g_server_destroy(g2g_server s);
 
 /* ------------------------------------------------------------ tuning */
 

@@ -1,0 +1,128 @@
+"""GPU: many transforms of mixed types and operations in one persistent
+launch (batches, hipGraph replay) and through the serving loop; every result
+must equal the oracle's result for that transform alone."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import grid2grid_amd as g2g
+from grid2grid_amd import workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed, dtypes=(g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64, g2g.COMPLEX128), hi=200):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        init, fin, dtype, trans = workloads.random_request(rng, dtypes=dtypes, lo=8, hi=hi, p_trans=0.4)
+        if trans == "T" and dtype in (g2g.COMPLEX64, g2g.COMPLEX128) and rng.random() < 0.5:
+            trans = "C"
+        out.append((init, fin, dtype, trans))
+    return out
+
+
+class Case:
+    """Host arenas (random bytes), device copies, expected result and plan."""
+
+    def __init__(self, init, fin, dtype, trans, rng, scaled=False):
+        es = g2g.DTYPE_SIZE[dtype]
+        self.dtype, self.trans = dtype, trans
+        if scaled:
+            self.ia = np.zeros(init.arena_elems * es, dtype=np.uint8)
+            self.fa = np.zeros(fin.arena_elems * es, dtype=np.uint8)
+            O.fill(init, self.ia, dtype, int(rng.integers(1, 1 << 30)))
+            O.fill(fin, self.fa, dtype, int(rng.integers(1, 1 << 30)))
+            self.alpha, self.beta = {g2g.FLOAT32: (np.float32(1.5), np.float32(-0.5)),
+                                     g2g.FLOAT64: (0.25, 3.0),
+                                     g2g.COMPLEX64: (np.complex64(1 - 1j), np.complex64(0.5j)),
+                                     g2g.COMPLEX128: (2 + 1j, -1 + 0.5j)}[dtype]
+        else:
+            self.ia = rng.integers(0, 256, size=init.arena_elems * es, dtype=np.uint8)
+            self.fa = rng.integers(0, 256, size=fin.arena_elems * es, dtype=np.uint8)
+            self.alpha = self.beta = None
+        self.want = self.fa.copy()
+        O.transform(init, self.ia.copy(), fin, self.want, dtype, trans, self.alpha, self.beta)
+        self.ti = torch.from_numpy(self.ia).to("cuda")
+        self.tf = torch.from_numpy(self.fa).to("cuda")
+        self.plan = g2g.Plan(init, self.ti.data_ptr(), fin, self.tf.data_ptr(), dtype, trans)
+
+    def reset(self):
+        self.tf.copy_(torch.from_numpy(self.fa).to("cuda"))
+
+    def check(self, what):
+        np.testing.assert_array_equal(self.tf.cpu().numpy(), self.want, err_msg=what)
+
+
+def _cases(n, seed, scaled_every=0):
+    rng = np.random.default_rng(seed)
+    reqs = _requests(n, seed)
+    return [Case(i, f, d, t, rng, scaled=bool(scaled_every) and k % scaled_every == 0)
+            for k, (i, f, d, t) in enumerate(reqs)]
+
+
+def test_batch_mixed_equals_alone():
+    cases = _cases(24, 1, scaled_every=5)
+    batch = g2g.Batch([c.plan for c in cases], [c.alpha for c in cases], [c.beta for c in cases])
+    assert batch.bytes == sum(c.plan.info.bytes for c in cases)
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, c in enumerate(cases):
+        c.check(f"request {k} dtype {c.dtype} trans {c.trans}")
+    batch.close()
+
+
+def test_batch_graph_replays():
+    """The captured graph gives the eager result on every replay, with eager
+    work and synchronizations between the replays."""
+    cases = _cases(16, 2)
+    stream = torch.cuda.Stream()
+    batch = g2g.Batch([c.plan for c in cases], stream=stream)
+    for rep in range(4):
+        for c in cases:
+            c.reset()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            batch.execute_graph(stream)
+        stream.synchronize()
+        for k, c in enumerate(cases):
+            c.check(f"replay {rep} request {k}")
+        # unrelated eager work between replays
+        x = torch.randn(1 << 20, device="cuda")
+        assert torch.isfinite(x.sum())
+    batch.close()
+
+
+def test_server_batched_results_equal_alone():
+    """Requests submitted while others run are admitted as budget frees up;
+    each result is identical to the transform run alone."""
+    cases = _cases(40, 3, scaled_every=7)
+    budget = max(2 * c.plan.info.bytes for c in cases) * 3  # forces waiting
+    srv = g2g.Server(budget, n_streams=3, max_batch=8)
+    tickets = [srv.submit(c.plan, c.alpha, c.beta) for c in cases]
+    lat = [srv.wait(t) for t in tickets]
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(cases) and st["rejected"] == 0
+    assert st["peak_in_flight_bytes"] <= budget
+    assert all(x >= 0 for x in lat)
+    for k, c in enumerate(cases):
+        c.check(f"served request {k}")
+
+
+def test_server_plain_mode_and_budget_reject():
+    cases = _cases(6, 4)
+    srv = g2g.Server(1 << 40, mode=g2g.SERVE_PLAIN)
+    for c in cases:
+        srv.wait(srv.submit(c.plan))
+    for k, c in enumerate(cases):
+        c.check(f"plain request {k}")
+    srv.close()
+    small = g2g.Server(16)
+    with pytest.raises(g2g.G2GError) as e:
+        small.submit(cases[0].plan)
+    assert e.value.code == 4
+    assert small.stats()["rejected"] == 1
+    small.close()

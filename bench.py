@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline leg")
+    ap.add_argument("--batch-requests", type=int, default=2048)
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--qps", type=float, default=2000.0)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     return ap.parse_args()
 
 
@@ -131,6 +135,159 @@ def cpu_baseline(budget_s):
     }
 
 
+def _percentile(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def make_requests(torch, g2g, reqs, seed0, dev):
+    """Resident buffers and plans for a list of (init, final, dtype, trans)."""
+    out = []
+    for k, (init, fin, dtype, trans) in enumerate(reqs):
+        es = g2g.DTYPE_SIZE[dtype]
+        ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device=dev)
+        tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device=dev)
+        g2g.fill_layout(init, ti.data_ptr(), dtype, seed0 + k)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+        out.append(dict(init=init, fin=fin, dtype=dtype, trans=trans, ti=ti, tf=tf, plan=plan, seed=seed0 + k,
+                        bytes=2 * fin.m * fin.n * es))
+    return out
+
+
+def verify_requests(g2g, rs):
+    bad = 0
+    for r in rs:
+        bad += g2g.verify_layout(r["fin"], r["tf"].data_ptr(), r["dtype"], r["seed"], r["trans"], r["init"].n)
+    return bad
+
+
+def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev):
+    """Config 4: n_req mixed transforms in one persistent launch (hipGraph
+    replay) versus plain per-request launches (the staged form per request)
+    and fused per-request launches."""
+    t0 = time.time()
+    rs = make_requests(torch, g2g, workloads.cfg4(n=n_req), 40000, dev)
+    nbytes = sum(r["bytes"] for r in rs)
+    log(f"cfg4: {n_req} requests, {nbytes / 1e9:.2f} GB moved per pass, planned in {time.time() - t0:.1f}s")
+    batch = g2g.Batch([r["plan"] for r in rs], stream=stream)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ms_b, _ = time_steps(torch, lambda: batch.execute_graph(s), steps, 2, s)
+    bad_b = verify_requests(g2g, rs)
+    for r in rs:
+        r["tf"].zero_()
+    ms_f, _ = time_steps(torch, lambda: [r["plan"].transform(stream=stream) for r in rs], max(2, steps // 2), 1,
+                         stream)
+    bad_f = verify_requests(g2g, rs)
+    for r in rs:
+        r["tf"].zero_()
+    for r in rs:  # allocate the staging buffers outside the timed region
+        r["plan"].transform_staged(stream=stream)
+    ms_p, _ = time_steps(torch, lambda: [r["plan"].transform_staged(stream=stream) for r in rs],
+                         max(2, steps // 2), 1, stream)
+    bad_p = verify_requests(g2g, rs)
+    batch.close()
+    out = {
+        "workload": f"{n_req} mixed transforms (m,n log-uniform [64,4096], f32/f64/c64, blocks 16..256, "
+                    f"grids 1x1..4x4, 30% transposed)",
+        "bytes_per_pass": int(nbytes),
+        "batched_graph": {"ms": round(ms_b, 3), "GB/s": round(nbytes / ms_b / 1e6, 1), "bit_exact": bad_b == 0},
+        "plain_per_request": {"ms": round(ms_p, 3), "GB/s": round(nbytes / ms_p / 1e6, 1), "bit_exact": bad_p == 0},
+        "fused_per_request": {"ms": round(ms_f, 3), "GB/s": round(nbytes / ms_f / 1e6, 1), "bit_exact": bad_f == 0},
+        "speedup_vs_plain": round(ms_p / ms_b, 2),
+        "speedup_vs_fused_per_request": round(ms_f / ms_b, 2),
+    }
+    for r in rs:
+        r["plan"].close()
+    del rs
+    torch.cuda.empty_cache()
+    return out
+
+
+def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev):
+    """Config 5: Poisson arrivals at `qps` of config-4 requests plus 5% large
+    16384^2 f64 requests, served with continuous batching under an HBM budget,
+    versus plain per-request launches; then the saturated throughput of both."""
+    import numpy as np
+    rng = np.random.default_rng(1005)
+    t0 = time.time()
+    small = make_requests(torch, g2g, [workloads.random_request(rng) for _ in range(n_small)], 50000, dev)
+    large = make_requests(torch, g2g, [workloads.large_request() for _ in range(n_large)], 60000, dev)
+    log(f"cfg5: pool of {n_small} small + {n_large} large requests ready in {time.time() - t0:.1f}s")
+    for r in small + large:  # staging buffers of the plain form, outside the timed region
+        r["plan"].transform_staged()
+    torch.cuda.synchronize()
+    # arrival sequence: 5% large
+    kinds = rng.random(n_req) < 0.05
+    gaps = rng.exponential(1.0 / qps, size=n_req)
+
+    def run(server, open_loop):
+        busy = {}  # pool index -> ticket in flight
+        pools = {True: list(range(len(large))), False: list(range(len(small)))}
+        nxt = {True: 0, False: 0}
+        lat, done_bytes = [], 0
+        t_start = time.perf_counter()
+        t_next = t_start
+        tickets = []
+        for i in range(n_req):
+            big = bool(kinds[i])
+            pool = large if big else small
+            # next pool entry not in flight
+            for _ in range(len(pool)):
+                j = pools[big][nxt[big] % len(pool)]
+                nxt[big] += 1
+                key = (big, j)
+                t = busy.get(key)
+                if t is None or server.poll(t) is not None:
+                    break
+            else:
+                server.wait(busy[key])
+            if open_loop:
+                t_next += gaps[i]
+                while time.perf_counter() < t_next:
+                    pass
+            t = server.submit(pool[j]["plan"])
+            busy[key] = t
+            tickets.append((t, pool[j]["bytes"]))
+        for t, b in tickets:
+            lat.append(server.wait(t))
+            done_bytes += b
+        wall = time.perf_counter() - t_start
+        return lat, done_bytes, wall
+
+    res = {}
+    for name, mode, streams, mb in (("served_batched", g2g.SERVE_BATCHED, 3, 512), ("plain_per_request", g2g.SERVE_PLAIN, 1, 1)):
+        srv = g2g.Server(budget, n_streams=streams, max_batch=mb, mode=mode)
+        lat, nbytes, wall = run(srv, True)
+        st = srv.stats()
+        srv.close()
+        srv = g2g.Server(budget, n_streams=streams, max_batch=mb, mode=mode)
+        _, sat_bytes, sat_wall = run(srv, False)
+        srv.close()
+        res[name] = {
+            "p50_ms": round(_percentile(lat, 50) * 1e3, 3),
+            "p99_ms": round(_percentile(lat, 99) * 1e3, 3),
+            "sustained_GB/s_at_qps": round(nbytes / wall / 1e9, 1),
+            "saturated_GB/s": round(sat_bytes / sat_wall / 1e9, 1),
+            "launches": int(st["launches"]),
+            "peak_in_flight_GB": round(st["peak_in_flight_bytes"] / 1e9, 2),
+        }
+        log(f"{name}: {res[name]}")
+    bad = verify_requests(g2g, small + large)
+    out = {
+        "workload": f"Poisson {qps} req/s, {n_req} requests (config-4 distribution + 5% 16384^2 f64), "
+                    f"HBM budget {budget / 1e9:.0f} GB",
+        **res,
+        "throughput_ratio_saturated": round(res["served_batched"]["saturated_GB/s"] /
+                                            max(1e-9, res["plain_per_request"]["saturated_GB/s"]), 2),
+        "bit_exact": bad == 0,
+    }
+    for r in small + large:
+        r["plan"].close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -218,6 +375,12 @@ def main():
     plan.close()
     del ti, tf
     torch.cuda.empty_cache()
+
+    if not args.no_extras:
+        result["batched"] = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev)
+        log(f"batched: {result['batched']}")
+        result["served"] = served_leg(torch, g2g, workloads, 512, 6, args.serve_requests, args.qps,
+                                      int(args.budget_gb * 1e9), dev)
 
     if not args.no_cpu_baseline:
         try:

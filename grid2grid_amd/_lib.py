@@ -46,6 +46,12 @@ class G2GPlanInfo(ctypes.Structure):
                 ("trans", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class G2GTuning(ctypes.Structure):
+    _fields_ = [("tile_bytes", ctypes.c_int64), ("threads", ctypes.c_int32),
+                ("unroll_bytes", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
+                ("persistent", ctypes.c_int32), ("ms", ctypes.c_double)]
+
+
 class G2GServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
                 ("rejected", ctypes.c_int64), ("launches", ctypes.c_int64),
@@ -86,6 +92,9 @@ PROTOTYPES = {
     "g2g_batch_bytes": (_i, [_vp, _p64]),
     "g2g_batch_destroy": (_i, [_vp]),
     "g2g_set_tuning": (_i, [_i64, _i]),
+    "g2g_plan_get_tuning": (_i, [_vp, ctypes.POINTER(G2GTuning)]),
+    "g2g_plan_set_tuning": (_i, [_vp, ctypes.POINTER(G2GTuning)]),
+    "g2g_plan_autotune": (_i, [_vp, _vp, _i, ctypes.POINTER(G2GTuning)]),
     "g2g_server_create": (_i, [_i64, _i, _i, _i, ctypes.POINTER(_vp)]),
     "g2g_server_submit": (_i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
     "g2g_server_poll": (_i, [_vp, _i64, _p32, ctypes.POINTER(ctypes.c_double)]),

@@ -104,6 +104,21 @@ class Plan:
             out["send_off"].ctypes.data_as(P64), out["recv_off"].ctypes.data_as(P64), n))
         return out
 
+    @property
+    def tuning(self):
+        t = _lib.G2GTuning()
+        check(_lib.load().g2g_plan_get_tuning(self.handle, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, persistent=0):
+        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(persistent), 0.0)
+        check(_lib.load().g2g_plan_set_tuning(self.handle, ctypes.byref(t)))
+
+    def autotune(self, stream=None, reps=3):
+        t = _lib.G2GTuning()
+        check(_lib.load().g2g_plan_autotune(self.handle, _stream(stream), reps, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def send_buffer(self, rank):
         p = ctypes.c_void_p()
         n = ctypes.c_int64()

@@ -239,6 +239,7 @@ struct stage_table {
     uint64_t total = 0;
     uint64_t tile_bytes = 0;
     uint32_t n_tiles = 0;
+    launch_cfg cfg;
 };
 
 struct g2g_plan_s {
@@ -250,6 +251,8 @@ struct g2g_plan_s {
     std::vector<span> urows, ucols;  // host copy of the interval table
 
     stage_table fused;               // all pieces, fused gather
+    uint64_t class_key = 0;          // shape class of the tuning table
+    double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
     dbuf<uint32_t> ranks;            // (src << 16) | dst per piece
     dbuf<uint32_t> queue;
 
@@ -276,6 +279,69 @@ bool persistent_mode(uint32_t n_tiles) {
 
 int grid_limit() { return device_cus() * 8; }
 
+// Work items of a table for a given size: first piece per item.
+int retile(stage_table &t, uint64_t tile_bytes, hipStream_t s) {
+    t.tile_bytes = tile_bytes;
+    const uint64_t nt = (t.total + t.tile_bytes - 1) / t.tile_bytes;
+    if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
+    t.n_tiles = (uint32_t)nt;
+    G2G_TRY(t.first.alloc(nt));
+    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, t.first.p, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+// ---- tuning table: best configuration per shape class
+struct tuned_cfg {
+    uint64_t tile_bytes;
+    launch_cfg cfg;
+    double ms;
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
+
+int ilog2(uint64_t v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+// element size, transposition, size class and piece-size class
+uint64_t shape_class(uint32_t es, char trans, uint64_t total, int64_t n_pieces) {
+    const uint64_t avg = n_pieces > 0 ? total / (uint64_t)n_pieces : total;
+    return ((uint64_t)es << 24) | ((uint64_t)(trans != 'N') << 16) | ((uint64_t)ilog2(total | 1) << 8) |
+           (uint64_t)ilog2(avg | 1);
+}
+
+bool lookup_tuned(uint64_t key, tuned_cfg &out) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (auto &e : g_tune_table)
+        if (e.first == key) {
+            out = e.second;
+            return true;
+        }
+    return false;
+}
+
+void store_tuned(uint64_t key, const tuned_cfg &c) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (auto &e : g_tune_table)
+        if (e.first == key) {
+            e.second = c;
+            return;
+        }
+    g_tune_table.push_back({key, c});
+}
+
+launch_cfg default_cfg(uint32_t n_tiles) {
+    launch_cfg c;
+    c.persistent = persistent_mode(n_tiles);
+    return c;
+}
+
 // bytes -> inclusive scan -> first piece per tile for a stage table whose
 // pieces are already on the device.
 int finalize_table(stage_table &t, const kreqs &kr, uint64_t total_hint, hipStream_t s) {
@@ -291,13 +357,8 @@ int finalize_table(stage_table &t, const kreqs &kr, uint64_t total_hint, hipStre
         return fail(G2G_ERR_INVALID, "internal: piece bytes %llu != matrix bytes %llu",
                     (unsigned long long)total, (unsigned long long)total_hint);
     t.total = total;
-    t.tile_bytes = choose_tile_bytes(total);
-    const uint64_t nt = (total + t.tile_bytes - 1) / t.tile_bytes;
-    if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
-    t.n_tiles = (uint32_t)nt;
-    G2G_TRY(t.first.alloc(nt));
-    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, t.first.p, s));
-    G2G_HIP(hipStreamSynchronize(s));
+    G2G_TRY(retile(t, choose_tile_bytes(total), s));
+    t.cfg = default_cfg(t.n_tiles);
     return G2G_OK;
 }
 
@@ -463,7 +524,9 @@ int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
     a.one = make_desc(t, kr, 0);
     a.total_tiles = t.n_tiles;
     a.queue = p->queue.p;
-    G2G_HIP(launch_gather(a, persistent_mode(t.n_tiles), grid_limit(), s));
+    launch_cfg c = t.cfg;
+    if (g_persistent.load() >= 0) c.persistent = g_persistent.load() != 0;
+    G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
 }
 
@@ -545,18 +608,22 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     G2G_HIP(inclusive_scan_u64(p->fused.pend.p, np, scratch.p, s));
     const uint64_t total = (uint64_t)p->m * p->n * es;
     p->fused.total = total;
-    p->fused.tile_bytes = choose_tile_bytes(total);
-    const uint64_t nt = (total + p->fused.tile_bytes - 1) / p->fused.tile_bytes;
-    if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
-    p->fused.n_tiles = (uint32_t)nt;
-    G2G_TRY(p->fused.first.alloc(nt));
-    G2G_HIP(launch_first_piece(p->fused.pend.p, np, p->fused.tile_bytes, p->fused.first.p, s));
     uint64_t check = 0;
     if (np) G2G_HIP(hipMemcpyAsync(&check, p->fused.pend.p + np - 1, sizeof check, hipMemcpyDeviceToHost, s));
     G2G_HIP(hipStreamSynchronize(s));
     if (check != total)
         return fail(G2G_ERR_INVALID, "internal: pieces cover %llu bytes, matrix has %llu",
                     (unsigned long long)check, (unsigned long long)total);
+    // configuration: the tuned one of this shape class, else the defaults
+    p->class_key = shape_class(es, trans, total, np);
+    tuned_cfg tc;
+    if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
+        G2G_TRY(retile(p->fused, tc.tile_bytes, s));
+        p->fused.cfg = tc.cfg;
+    } else {
+        G2G_TRY(retile(p->fused, choose_tile_bytes(total), s));
+        p->fused.cfg = default_cfg(p->fused.n_tiles);
+    }
     *out = p;
     p = nullptr;  // released from the guard
     return G2G_OK;
@@ -564,6 +631,95 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
 
 int g2g_plan_destroy(g2g_plan plan) {
     delete plan;
+    return G2G_OK;
+}
+
+int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
+    if (!p || !t) return fail(G2G_ERR_INVALID, "NULL argument");
+    t->tile_bytes = (int64_t)p->fused.tile_bytes;
+    t->threads = p->fused.cfg.threads;
+    t->unroll_bytes = p->fused.cfg.unroll_bytes;
+    t->nontemporal = p->fused.cfg.nontemporal;
+    t->persistent = p->fused.cfg.persistent;
+    t->ms = p->tuned_ms;
+    return G2G_OK;
+}
+
+int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
+    if (!p || !t) return fail(G2G_ERR_INVALID, "NULL argument");
+    if (t->tile_bytes < 4096 || t->tile_bytes > (1ll << 26) || (t->threads != 256 && t->threads != 512) ||
+        (t->unroll_bytes != 64 && t->unroll_bytes != 128))
+        return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
+    if ((uint64_t)t->tile_bytes != p->fused.tile_bytes) G2G_TRY(retile(p->fused, (uint64_t)t->tile_bytes, nullptr));
+    p->fused.cfg.threads = t->threads;
+    p->fused.cfg.unroll_bytes = t->unroll_bytes;
+    p->fused.cfg.nontemporal = t->nontemporal != 0;
+    p->fused.cfg.persistent = t->persistent != 0;
+    return G2G_OK;
+}
+
+int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    reps = std::max(1, reps);
+    hipEvent_t e0, e1;
+    G2G_HIP(hipEventCreate(&e0));
+    G2G_HIP(hipEventCreate(&e1));
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
+    kr.r[1] = kr.r[0];
+    const uint64_t total = p->fused.total;
+    std::vector<uint64_t> sizes;
+    for (uint64_t tb = 32768; tb <= (1u << 20); tb *= 2)
+        if (sizes.empty() || tb < total * 2) sizes.push_back(tb);
+    double best_ms = 1e30;
+    tuned_cfg win{p->fused.tile_bytes, p->fused.cfg, 0};
+    int rc = G2G_OK;
+    for (uint64_t tb : sizes) {
+        if ((rc = retile(p->fused, tb, s)) != G2G_OK) break;
+        for (int th : {256, 512})
+            for (int ub : {64, 128})
+                for (int nt = 0; nt < 2; ++nt)
+                    for (int pe = 0; pe < 2; ++pe) {
+                        launch_cfg c;
+                        c.threads = th;
+                        c.unroll_bytes = ub;
+                        c.nontemporal = nt != 0;
+                        c.persistent = pe != 0;
+                        p->fused.cfg = c;
+                        gather_args a;
+                        std::memset(&a, 0, sizeof a);
+                        a.one = make_desc(p->fused, kr, 0);
+                        a.total_tiles = p->fused.n_tiles;
+                        a.queue = p->queue.p;
+                        hipError_t he = launch_gather(a, c, device_cus(), s);  // warm-up
+                        if (he == hipSuccess) he = hipEventRecord(e0, s);
+                        for (int r = 0; r < reps && he == hipSuccess; ++r) he = launch_gather(a, c, device_cus(), s);
+                        if (he == hipSuccess) he = hipEventRecord(e1, s);
+                        if (he == hipSuccess) he = hipEventSynchronize(e1);
+                        float ms = 0;
+                        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+                        if (he != hipSuccess) {
+                            rc = fail(G2G_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+                            break;
+                        }
+                        const double per = ms / reps;
+                        if (per < best_ms) {
+                            best_ms = per;
+                            win = tuned_cfg{tb, c, per};
+                        }
+                    }
+        if (rc != G2G_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != G2G_OK) return rc;
+    G2G_TRY(retile(p->fused, win.tile_bytes, s));
+    p->fused.cfg = win.cfg;
+    p->tuned_ms = win.ms;
+    store_tuned(p->class_key, win);
+    if (best) G2G_TRY(g2g_plan_get_tuning(p, best));
     return G2G_OK;
 }
 
@@ -693,7 +849,9 @@ int batch_launch(g2g_batch b, hipStream_t s) {
     a.tile_owner = b->owner.p;
     a.total_tiles = b->total_tiles;
     a.queue = b->queue.p;
-    G2G_HIP(launch_gather(a, true, grid_limit(), s));
+    launch_cfg c;
+    c.persistent = true;
+    G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
 }
 
@@ -723,7 +881,7 @@ int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas, c
     return G2G_OK;
 }
 
-int internal_grid_limit() { return grid_limit(); }
+int internal_grid_limit() { return device_cus(); }
 
 int internal_fail(int code, const char *fmt, ...) {
     char buf[512];

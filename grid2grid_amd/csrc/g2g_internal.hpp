@@ -131,8 +131,16 @@ struct gather_args {
     uint32_t total_tiles;
     uint32_t *queue;              // persistent-mode work counter (reset by the launcher)
 };
-hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
-                         hipStream_t stream);
+// Compiled variants of the gather: workgroup size, bytes each thread keeps in
+// flight, non-temporal hint, launch mode.
+struct launch_cfg {
+    int threads = 256;       // 256 or 512
+    int unroll_bytes = 64;   // 64 or 128
+    bool nontemporal = false;
+    bool persistent = false;
+};
+// `cus` = compute units of the device (persistent grids fill them).
+hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream);
 
 // tile_owner[w] = k for w in [descs[k].tile_base, + n_tiles), k < n.
 hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_owner,
@@ -145,7 +153,7 @@ hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_o
 int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas,
                    const void *const *betas, std::vector<work_desc> &out, uint32_t &total_tiles,
                    uint64_t &total_bytes);
-// Workgroups of a persistent launch filling the device.
+// Compute units of the device (persistent grids are sized from it).
 int internal_grid_limit();
 // Records the thread's last error message and returns code.
 int internal_fail(int code, const char *fmt, ...);

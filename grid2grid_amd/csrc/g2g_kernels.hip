@@ -114,20 +114,37 @@ __device__ __forceinline__ uint32_t ceil_log2(uint32_t v) {
     return v <= 1 ? 0u : 32u - (uint32_t)__clz(v - 1);
 }
 
+// Global loads/stores, optionally with the non-temporal hint (streaming data
+// that is touched exactly once).
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st(T *p, T v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // Column-contiguous copy of ncols columns of vpc V-byte vectors each.
 // Lanes are grouped TX per column (TX = next power of two >= vpc, <= 64), so a
 // wave covers 64/TX columns of a short piece or 64 vectors of a long column.
-// Each thread keeps U = 64/V vectors (64 bytes) in flight before storing.
-template <int V>
+// Each thread keeps U = UB/V vectors (UB bytes) in flight before storing.
+template <int THREADS, int UB, bool NT, int V>
 __device__ __forceinline__ void copy_cols(const unsigned char *__restrict__ s,
                                           unsigned char *__restrict__ d, uint64_t lds_b,
                                           uint64_t ldd_b, uint32_t vpc, uint32_t ncols) {
     using T = typename bits_t<V>::type;
-    constexpr int U = 64 / V;
+    constexpr int U = UB / V;
     const uint32_t tid = threadIdx.x;
     const uint32_t txl = vpc >= 64 ? 6u : ceil_log2(vpc);
     const uint32_t TX = 1u << txl;
-    const uint32_t NY = (uint32_t)kThreads >> txl;
+    const uint32_t NY = (uint32_t)THREADS >> txl;
     const uint32_t tx = tid & (TX - 1);
     const uint32_t ty = tid >> txl;
     for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
@@ -136,19 +153,19 @@ __device__ __forceinline__ void copy_cols(const unsigned char *__restrict__ s,
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) r[u] = *(const T *)(s + c * lds_b + (uint64_t)v * V);
+                if (c < ncols) r[u] = ld<NT>((const T *)(s + c * lds_b + (uint64_t)v * V));
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) *(T *)(d + c * ldd_b + (uint64_t)v * V) = r[u];
+                if (c < ncols) st<NT>((T *)(d + c * ldd_b + (uint64_t)v * V), r[u]);
             }
         }
     }
 }
 
 // Scaled column copy, one element per lane slot.
-template <int ES>
+template <int THREADS, int ES>
 __device__ __forceinline__ void copy_cols_scaled(const unsigned char *__restrict__ s,
                                                  unsigned char *__restrict__ d, uint64_t lds_b,
                                                  uint64_t ldd_b, uint32_t rows, uint32_t ncols,
@@ -158,7 +175,7 @@ __device__ __forceinline__ void copy_cols_scaled(const unsigned char *__restrict
     const uint32_t tid = threadIdx.x;
     const uint32_t txl = rows >= 64 ? 6u : ceil_log2(rows);
     const uint32_t TX = 1u << txl;
-    const uint32_t NY = (uint32_t)kThreads >> txl;
+    const uint32_t NY = (uint32_t)THREADS >> txl;
     const uint32_t tx = tid & (TX - 1);
     const uint32_t ty = tid >> txl;
     for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
@@ -185,7 +202,7 @@ __device__ __forceinline__ void copy_cols_scaled(const unsigned char *__restrict
 // Source element (i, j) (final orientation) is at s + (i*lds + j)*ES, i.e.
 // storage rows run along final columns and are read contiguously; the
 // destination is written contiguously along final rows.
-template <int ES>
+template <int THREADS, bool NT, int ES>
 __device__ __forceinline__ void copy_transposed(const unsigned char *__restrict__ s,
                                                 unsigned char *__restrict__ d, uint64_t lds_b,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
@@ -193,8 +210,8 @@ __device__ __forceinline__ void copy_transposed(const unsigned char *__restrict_
     using T = typename bits_t<ES>::type;
     T(*tile)[kTileDim + 1] = reinterpret_cast<T(*)[kTileDim + 1]>(smem);
     const uint32_t x = threadIdx.x & (kTileDim - 1);
-    const uint32_t y = threadIdx.x / kTileDim;  // 0..7
-    constexpr uint32_t YS = kThreads / kTileDim;
+    const uint32_t y = threadIdx.x / kTileDim;
+    constexpr uint32_t YS = THREADS / kTileDim;
     const bool conj = (rq.flags & REQ_CONJ) != 0;
     const bool scale = (rq.flags & REQ_SCALE) != 0;
     for (uint32_t jc = 0; jc < ncols; jc += kTileDim) {
@@ -202,7 +219,7 @@ __device__ __forceinline__ void copy_transposed(const unsigned char *__restrict_
 #pragma unroll
             for (uint32_t k = 0; k < kTileDim; k += YS) {
                 const uint32_t i = ic + y + k, j = jc + x;
-                if (i < rows && j < ncols) tile[y + k][x] = *(const T *)(s + i * lds_b + (uint64_t)j * ES);
+                if (i < rows && j < ncols) tile[y + k][x] = ld<NT>((const T *)(s + i * lds_b + (uint64_t)j * ES));
             }
             __syncthreads();
 #pragma unroll
@@ -213,7 +230,7 @@ __device__ __forceinline__ void copy_transposed(const unsigned char *__restrict_
                     if (conj) v = conj_bits(v);
                     T *dp = (T *)(d + j * ldd_b + (uint64_t)i * ES);
                     if (scale) v = scale_bits(v, *dp, rq);
-                    *dp = v;
+                    st<NT>(dp, v);
                 }
             }
             __syncthreads();
@@ -221,6 +238,7 @@ __device__ __forceinline__ void copy_transposed(const unsigned char *__restrict_
     }
 }
 
+template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void process_piece(const piece &pc, const request &rq, uint32_t c0,
                                               uint32_t c1, unsigned char *smem) {
     const uint32_t es = rq.esize;
@@ -231,37 +249,38 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     if (rq.flags & REQ_TRANS) {
         const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * es;
         if (es == 4)
-            copy_transposed<4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else if (es == 8)
-            copy_transposed<8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else
-            copy_transposed<16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         return;
     }
     const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * lds_b;
     if (rq.flags & REQ_SCALE) {
         if (es == 4)
-            copy_cols_scaled<4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+            copy_cols_scaled<THREADS, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
         else if (es == 8)
-            copy_cols_scaled<8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+            copy_cols_scaled<THREADS, 8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
         else
-            copy_cols_scaled<16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+            copy_cols_scaled<THREADS, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
         return;
     }
     const uint32_t colb = pc.rows * es;
     const uint64_t al = (uint64_t)s | (uint64_t)d | colb | (ncols > 1 ? (lds_b | ldd_b) : 0);
     if ((al & 15) == 0)
-        copy_cols<16>(s, d, lds_b, ldd_b, colb >> 4, ncols);
+        copy_cols<THREADS, UB, NT, 16>(s, d, lds_b, ldd_b, colb >> 4, ncols);
     else if ((al & 7) == 0)
-        copy_cols<8>(s, d, lds_b, ldd_b, colb >> 3, ncols);
+        copy_cols<THREADS, UB, NT, 8>(s, d, lds_b, ldd_b, colb >> 3, ncols);
     else
-        copy_cols<4>(s, d, lds_b, ldd_b, colb >> 2, ncols);
+        copy_cols<THREADS, UB, NT, 4>(s, d, lds_b, ldd_b, colb >> 2, ncols);
 }
 
 // Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
 // stream (pieces concatenated in table order, each column-major). It moves
 // every column (every 32-column group for transposed pieces) whose first
 // byte lies in that range, so work items partition the columns exactly.
+template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
                                              unsigned char *smem) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
@@ -278,29 +297,30 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
         if (lo > pstart) c0 = ((lo - pstart + qb - 1) / qb) * q;
         uint64_t c1 = ((hi - pstart + qb - 1) / qb) * q;
         if (c1 > pc.cols) c1 = pc.cols;
-        if (c0 < c1) process_piece(pc, rq, (uint32_t)c0, (uint32_t)c1, smem);
+        if (c0 < c1) process_piece<THREADS, UB, NT>(pc, rq, (uint32_t)c0, (uint32_t)c1, smem);
     }
 }
 
 // Global work index w -> (transform, local work item) -> columns to move.
+template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
                                              unsigned char *smem) {
     if (!a.descs) {
-        process_tile(a.one, w, smem);
+        process_tile<THREADS, UB, NT>(a.one, w, smem);
         return;
     }
     const uint32_t k = a.tile_owner[w];
     const work_desc d = a.descs[k];
-    process_tile(d, w - d.tile_base, smem);
+    process_tile<THREADS, UB, NT>(d, w - d.tile_base, smem);
 }
 
 // Direct mode: one workgroup per work index. Persistent mode: a grid sized to
 // the chip pulls work indices from a device counter until all are taken.
-template <bool PERSIST>
-__global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
+template <bool PERSIST, int THREADS, int UB, bool NT>
+__global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[kTileDim * (kTileDim + 1) * 16];
     if (!PERSIST) {
-        process_work(a, blockIdx.x, smem);
+        process_work<THREADS, UB, NT>(a, blockIdx.x, smem);
         return;
     }
     __shared__ uint32_t s_t;
@@ -310,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
         if (t >= a.total_tiles) break;
-        process_work(a, t, smem);
+        process_work<THREADS, UB, NT>(a, t, smem);
     }
 }
 
@@ -318,17 +338,39 @@ __global__ __launch_bounds__(kThreads) void k_gather(gather_args a) {
 // captured hipGraph.
 __global__ void k_reset_queue(uint32_t *q) { *q = 0; }
 
-hipError_t launch_gather(const gather_args &a, bool persistent, int grid_limit,
-                         hipStream_t stream) {
-    if (a.total_tiles == 0) return hipSuccess;
-    if (!persistent) {
-        hipLaunchKernelGGL(k_gather<false>, dim3(a.total_tiles), dim3(kThreads), 0, stream, a);
+template <bool P, int TH, int UB, bool NT>
+static hipError_t launch_variant(const gather_args &a, int grid_limit, hipStream_t stream) {
+    if (!P) {
+        hipLaunchKernelGGL((k_gather<false, TH, UB, NT>), dim3(a.total_tiles), dim3(TH), 0, stream, a);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
     const uint32_t grid = std::min<uint32_t>(a.total_tiles, (uint32_t)std::max(1, grid_limit));
-    hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((k_gather<true, TH, UB, NT>), dim3(grid), dim3(TH), 0, stream, a);
     return hipGetLastError();
+}
+
+template <bool P, int TH, int UB>
+static hipError_t launch_nt(const gather_args &a, const launch_cfg &c, int gl, hipStream_t s) {
+    return c.nontemporal ? launch_variant<P, TH, UB, true>(a, gl, s) : launch_variant<P, TH, UB, false>(a, gl, s);
+}
+
+template <bool P, int TH>
+static hipError_t launch_ub(const gather_args &a, const launch_cfg &c, int gl, hipStream_t s) {
+    return c.unroll_bytes >= 128 ? launch_nt<P, TH, 128>(a, c, gl, s) : launch_nt<P, TH, 64>(a, c, gl, s);
+}
+
+template <bool P>
+static hipError_t launch_th(const gather_args &a, const launch_cfg &c, int gl, hipStream_t s) {
+    return c.threads >= 512 ? launch_ub<P, 512>(a, c, gl, s) : launch_ub<P, 256>(a, c, gl, s);
+}
+
+hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream) {
+    if (a.total_tiles == 0) return hipSuccess;
+    // persistent grids fill every CU at full occupancy (32 waves per CU)
+    const int per_cu = c.threads >= 512 ? 4 : 8;
+    const int gl = cus * per_cu;
+    return c.persistent ? launch_th<true>(a, c, gl, stream) : launch_th<false>(a, c, gl, stream);
 }
 
 __global__ void k_tile_owner(const work_desc *descs, uint32_t *owner) {

@@ -142,7 +142,9 @@ int g2g_server_s::launch(slot &sl, std::vector<pending_req> &batch) {
         a.tile_owner = sl.d_owner;
         a.total_tiles = total_tiles;
         a.queue = sl.d_queue;
-        if (launch_gather(a, true, internal_grid_limit(), sl.stream) != hipSuccess)
+        launch_cfg c;
+        c.persistent = true;
+        if (launch_gather(a, c, internal_grid_limit(), sl.stream) != hipSuccess)
             return internal_fail(G2G_ERR_HIP, "server: gather launch failed");
     }
     if (hipEventRecord(sl.done, sl.stream) != hipSuccess)

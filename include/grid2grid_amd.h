@@ -244,6 +244,28 @@ int g2g_server_destroy(g2g_server s);
  * tile_bytes in [4096, 1<<26] (0 = automatic), persistent 0/1 (-1 = automatic). */
 int g2g_set_tuning(int64_t tile_bytes, int persistent);
 
+/* Launch configuration of a plan's fused gather. */
+typedef struct g2g_tuning {
+    int64_t tile_bytes;    /* bytes of the piece stream per work item */
+    int32_t threads;       /* workgroup size: 256 or 512 */
+    int32_t unroll_bytes;  /* bytes each thread keeps in flight: 64 or 128 */
+    int32_t nontemporal;   /* 1: non-temporal loads/stores */
+    int32_t persistent;    /* 0: one workgroup per work item; 1: persistent grid + device queue */
+    double ms;             /* measured time per transform (set by g2g_plan_autotune) */
+} g2g_tuning;
+
+int g2g_plan_get_tuning(g2g_plan plan, g2g_tuning *t);
+int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
+
+/* Autotuner: time every compiled variant (workgroup size, bytes in flight,
+ * non-temporal hint, launch mode) at several work-item sizes on this plan
+ * (`reps` timed runs each, on `stream`, hipEvents), keep the fastest for the
+ * plan and record it for the plan's shape class (element size, operation,
+ * size class, piece-size class): plans created later in this process with
+ * the same class start from it. The transform runs while tuning (the final
+ * layout is written). *best receives the chosen configuration. */
+int g2g_plan_autotune(g2g_plan plan, void *stream, int reps, g2g_tuning *best);
+
 /* ------------------------------------------------ test-support kernels */
 
 /* Fill every block of `layout` with the seeded value v(seed, i, j) of its

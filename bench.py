@@ -30,7 +30,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "copy"])
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
+    ap.add_argument("--tuning", default=None,
+                    help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:persistent "
+                         "(skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -48,6 +52,12 @@ def log(*a):
 
 def build_workload(name):
     from grid2grid_amd import workloads
+    if name == "copy":
+        # one 32768^2 f64 block on each side: a plain contiguous 8 GiB copy
+        # (reference point for the HBM byte counters)
+        from grid2grid_amd.layout import scalapack_layout
+        lay = scalapack_layout(32768, 32768, 32768, 32768, 1, 1, "C", lld="rows")
+        return lay, scalapack_layout(32768, 32768, 32768, 32768, 1, 1, "C", lld="rows"), 1, "N"
     return {"cfg1": workloads.cfg1, "cfg2": workloads.cfg2, "cfg3": workloads.cfg3}[name]()
 
 
@@ -55,6 +65,7 @@ WORKLOAD_DESC = {
     "cfg1": "block-cyclic 4096^2 f64, 32x32 on 2x2 -> 64x128 on 2x2, 4 virtual ranks",
     "cfg2": "block-cyclic 32768^2 f64, 128x128 on 4x4 -> 256x512 on 2x8, 16 virtual ranks",
     "cfg3": "irregular 40000x25000 f32, 64x48 -> 50x40 splits, 32 virtual ranks, padded ld",
+    "copy": "single-block 32768^2 f64 -> single block: plain contiguous 8 GiB copy (calibration)",
 }
 
 
@@ -315,6 +326,17 @@ def main():
     info = plan.info
     log(f"plan: {info.n_pieces} pieces, {info.n_tiles} work items of {info.tile_bytes} B, "
         f"planned in {time.time() - t0:.2f}s")
+    tuning = plan.tuning
+    if args.tuning:
+        tb, th, ub, nt, pe = (int(x) for x in args.tuning.split(":"))
+        plan.set_tuning(tb, th, ub, nt, pe)
+        tuning = plan.tuning
+        info = plan.info
+    elif args.mode == "fused" and not args.no_autotune:
+        t0 = time.time()
+        tuning = plan.autotune(stream=stream, reps=3)
+        info = plan.info
+        log(f"autotune ({time.time() - t0:.1f}s): {tuning}")
 
     run = {"fused": lambda: plan.transform(stream=stream),
            "staged": lambda: plan.transform_staged(stream=stream)}[args.mode]
@@ -348,6 +370,7 @@ def main():
             "pieces": int(info.n_pieces),
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
+            "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "persistent")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

@@ -477,14 +477,26 @@ int ensure_staged(g2g_plan p, hipStream_t s) {
             uint64_t cnt = p->counts[(size_t)sr * P + d];
             uint64_t so = p->sbase[sr] + p->sdspl[(size_t)sr * P + d];
             uint64_t ro = p->rbase[d] + p->rdspl[(size_t)d * P + sr];
+            // A contiguous segment viewed as R-element columns (any factoring
+            // of a contiguous range is the same copy), so the gather spreads it
+            // over whole workgroups; the remainder is one short column.
+            const uint64_t R = 1024;
             while (cnt) {
-                const uint64_t len = std::min<uint64_t>(cnt, 1u << 30);
                 piece pc;
                 std::memset(&pc, 0, sizeof pc);
                 pc.src = send0 + so * es;
                 pc.dst = recv0 + ro * es;
-                pc.lds = pc.ldd = pc.rows = (uint32_t)len;
-                pc.cols = 1;
+                uint64_t len;
+                if (cnt >= R) {
+                    const uint64_t cols = std::min<uint64_t>(cnt / R, 1u << 30);
+                    pc.lds = pc.ldd = pc.rows = (uint32_t)R;
+                    pc.cols = (uint32_t)cols;
+                    len = cols * R;
+                } else {
+                    pc.lds = pc.ldd = pc.rows = (uint32_t)cnt;
+                    pc.cols = 1;
+                    len = cnt;
+                }
                 seg.push_back(pc);
                 cnt -= len;
                 so += len;

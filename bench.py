@@ -30,7 +30,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "copy"])
+    ap.add_argument("--workload", default="cfg2",
+                    help="cfg1 | cfg2 | cfg3 | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
                     help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:persistent "
@@ -58,6 +59,16 @@ def build_workload(name):
         from grid2grid_amd.layout import scalapack_layout
         lay = scalapack_layout(32768, 32768, 32768, 32768, 1, 1, "C", lld="rows")
         return lay, scalapack_layout(32768, 32768, 32768, 32768, 1, 1, "C", lld="rows"), 1, "N"
+    if name.startswith("bc:"):
+        # bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans -- any block-cyclic pair
+        f = name.split(":")[1:]
+        m, n, imb, inb, ipr, ipc, fmb, fnb, fpr, fpc = (int(x) for x in f[:10])
+        from grid2grid_amd import DTYPE_NAMES
+        from grid2grid_amd.layout import scalapack_layout
+        dtype, trans = DTYPE_NAMES[f[10]], f[11]
+        fm, fn = (n, m) if trans in "TC" else (m, n)
+        return (scalapack_layout(m, n, imb, inb, ipr, ipc, "C", lld="rows"),
+                scalapack_layout(fm, fn, fmb, fnb, fpr, fpc, "C", lld="rows"), dtype, trans)
     return {"cfg1": workloads.cfg1, "cfg2": workloads.cfg2, "cfg3": workloads.cfg3}[name]()
 
 
@@ -317,7 +328,7 @@ def main():
 
     init, fin, dtype, trans = build_workload(args.workload)
     es = g2g.DTYPE_SIZE[dtype]
-    log(f"workload {args.workload}: {WORKLOAD_DESC[args.workload]}")
+    log(f"workload {args.workload}: {WORKLOAD_DESC.get(args.workload, args.workload)}")
     ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device=dev)
     tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device=dev)
     g2g.fill_layout(init, ti.data_ptr(), dtype, 2, stream)
@@ -363,7 +374,7 @@ def main():
         "dtype": {g2g.FLOAT32: "f32", g2g.FLOAT64: "f64", g2g.COMPLEX64: "c64", g2g.COMPLEX128: "c128"}[dtype],
         "data": "synthetic (seeded splitmix64 values in [-1,1))",
         "config": {
-            "workload": WORKLOAD_DESC[args.workload],
+            "workload": WORKLOAD_DESC.get(args.workload, args.workload),
             "mode": args.mode,
             "m": fin.m, "n": fin.n,
             "virtual_ranks": int(max(init.n_ranks, fin.n_ranks)),

@@ -11,8 +11,7 @@ struct g2g_plan_s;
 
 namespace g2g {
 
-constexpr int kThreads = 256;    // threads per workgroup of every data-movement kernel
-constexpr int kTileDim = 32;     // transpose tile edge (elements)
+constexpr int kThreads = 256;    // threads per workgroup of the planning kernels
 
 enum : uint32_t {
     REQ_TRANS = 1u,  // destination(i, j) = source(j, i)

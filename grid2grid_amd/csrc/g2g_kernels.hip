@@ -12,6 +12,17 @@ namespace g2g {
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// A 16-byte vector at an address that is only 4-byte aligned. HSA code
+// objects run in unaligned-access mode, where global vector accesses need
+// dword alignment only.
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// Pointers into global memory (address space 1): the copy loops use global_*
+// instructions rather than flat_* ones, which would also hold the LDS/scalar
+// wait counter.
+#define GLOBAL __attribute__((address_space(1)))
+typedef GLOBAL unsigned char gbyte;
+
 template <int B> struct bits_t;
 template <> struct bits_t<4> { using type = uint32_t; };
 template <> struct bits_t<8> { using type = u32x2; };
@@ -117,48 +128,136 @@ __device__ __forceinline__ uint32_t ceil_log2(uint32_t v) {
 // Global loads/stores, optionally with the non-temporal hint (streaming data
 // that is touched exactly once).
 template <bool NT, typename T>
-__device__ __forceinline__ T ld(const T *p) {
+__device__ __forceinline__ T ld(const GLOBAL T *p) {
     if constexpr (NT)
         return __builtin_nontemporal_load(p);
     else
         return *p;
 }
 template <bool NT, typename T>
-__device__ __forceinline__ void st(T *p, T v) {
+__device__ __forceinline__ void st(GLOBAL T *p, T v) {
     if constexpr (NT)
         __builtin_nontemporal_store(v, p);
     else
         *p = v;
 }
 
-// Column-contiguous copy of ncols columns of vpc V-byte vectors each.
-// Lanes are grouped TX per column (TX = next power of two >= vpc, <= 64), so a
-// wave covers 64/TX columns of a short piece or 64 vectors of a long column.
-// Each thread keeps U = UB/V vectors (UB bytes) in flight before storing.
+// Unsigned 32-bit division by a divisor fixed for a whole piece, as a
+// multiply-high and shifts (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", 1994, fig. 4.1).
+struct fastdiv {
+    uint32_t mul, s1, s2;
+    __device__ __forceinline__ explicit fastdiv(uint32_t d) {
+        const uint32_t l = ceil_log2(d);
+        mul = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+        s1 = l > 0 ? 1u : 0u;
+        s2 = l > 0 ? l - 1 : 0u;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        const uint32_t t = __umulhi(mul, n);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
+// 16-byte accesses at dword-aligned addresses.
+template <bool NT>
+__device__ __forceinline__ u32x4 ld_a4(const gbyte *p) {
+    const GLOBAL u32x4a4 *q = (const GLOBAL u32x4a4 *)p;
+    if constexpr (NT)
+        return __builtin_nontemporal_load(q);
+    else
+        return *q;
+}
+template <bool NT>
+__device__ __forceinline__ void st_a4(gbyte *p, u32x4 v) {
+    GLOBAL u32x4a4 *q = (GLOBAL u32x4a4 *)p;
+    const u32x4a4 w = v;
+    if constexpr (NT)
+        __builtin_nontemporal_store(w, q);
+    else
+        *q = w;
+}
+
+// Column-contiguous copy of ncols columns of vpc V-byte vectors each. The
+// ncols*vpc vector slots are numbered column by column and dealt to the
+// threads in order, so consecutive lanes read consecutive vectors of a column
+// (a wave spans several short columns, or one stretch of a long one) and every
+// thread keeps U = UB/V vectors (UB bytes) in flight whatever the piece shape.
 template <int THREADS, int UB, bool NT, int V>
-__device__ __forceinline__ void copy_cols(const unsigned char *__restrict__ s,
-                                          unsigned char *__restrict__ d, uint64_t lds_b,
+__device__ __forceinline__ void copy_cols(const gbyte *__restrict__ s,
+                                          gbyte *__restrict__ d, uint64_t lds_b,
                                           uint64_t ldd_b, uint32_t vpc, uint32_t ncols) {
     using T = typename bits_t<V>::type;
     constexpr int U = UB / V;
     const uint32_t tid = threadIdx.x;
-    const uint32_t txl = vpc >= 64 ? 6u : ceil_log2(vpc);
-    const uint32_t TX = 1u << txl;
-    const uint32_t NY = (uint32_t)THREADS >> txl;
-    const uint32_t tx = tid & (TX - 1);
-    const uint32_t ty = tid >> txl;
-    for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
-        for (uint32_t v = tx; v < vpc; v += TX) {
-            T r[U];
+    const uint32_t slots = ncols * vpc;
+    const fastdiv per_col(vpc);
+    for (uint32_t base = tid; base < slots; base += (uint32_t)THREADS * U) {
+        T r[U];
+        uint64_t so[U], dof[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) r[u] = ld<NT>((const T *)(s + c * lds_b + (uint64_t)v * V));
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = base + (uint32_t)u * THREADS;
+            const uint32_t c = per_col.div(k);
+            const uint32_t v = k - c * vpc;
+            so[u] = c * lds_b + (uint64_t)v * V;
+            dof[u] = c * ldd_b + (uint64_t)v * V;
+            if (k < slots) r[u] = ld<NT>((const GLOBAL T *)(s + so[u]));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (base + (uint32_t)u * THREADS < slots) st<NT>((GLOBAL T *)(d + dof[u]), r[u]);
+        }
+    }
+}
+
+// The same copy for columns that are only 4-byte aligned (padded leading
+// dimensions, blocks at arbitrary element offsets): 16-byte accesses at
+// dword-aligned addresses, and the last 1-3 dwords of a column whose byte
+// length is not a multiple of 16 moved one by one.
+template <int THREADS, int UB, bool NT>
+__device__ __forceinline__ void copy_cols_dw16(const gbyte *__restrict__ s,
+                                               gbyte *__restrict__ d, uint64_t lds_b,
+                                               uint64_t ldd_b, uint32_t colb, uint32_t ncols) {
+    constexpr int U = UB / 16;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t vpc = (colb + 15) >> 4;
+    const uint32_t tail = (colb & 15) >> 2;  // dwords of a partial last vector (0: none)
+    const uint32_t slots = ncols * vpc;
+    const fastdiv per_col(vpc);
+    for (uint32_t base = tid; base < slots; base += (uint32_t)THREADS * U) {
+        u32x4 r[U];
+        uint64_t so[U], dof[U];
+        bool part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = base + (uint32_t)u * THREADS;
+            const uint32_t c = per_col.div(k);
+            const uint32_t v = k - c * vpc;
+            so[u] = c * lds_b + (uint64_t)v * 16;
+            dof[u] = c * ldd_b + (uint64_t)v * 16;
+            part[u] = tail != 0 && v + 1 == vpc;
+            if (k < slots) {
+                if (!part[u]) {
+                    r[u] = ld_a4<NT>(s + so[u]);
+                } else {
+                    const GLOBAL uint32_t *p = (const GLOBAL uint32_t *)(s + so[u]);
+                    r[u].x = p[0];
+                    r[u].y = tail > 1 ? p[1] : 0u;
+                    r[u].z = tail > 2 ? p[2] : 0u;
+                }
             }
+        }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) st<NT>((T *)(d + c * ldd_b + (uint64_t)v * V), r[u]);
+        for (int u = 0; u < U; ++u) {
+            if (base + (uint32_t)u * THREADS >= slots) continue;
+            if (!part[u]) {
+                st_a4<NT>(d + dof[u], r[u]);
+            } else {
+                GLOBAL uint32_t *p = (GLOBAL uint32_t *)(d + dof[u]);
+                p[0] = r[u].x;
+                if (tail > 1) p[1] = r[u].y;
+                if (tail > 2) p[2] = r[u].z;
             }
         }
     }
@@ -166,71 +265,140 @@ __device__ __forceinline__ void copy_cols(const unsigned char *__restrict__ s,
 
 // Scaled column copy, one element per lane slot.
 template <int THREADS, int ES>
-__device__ __forceinline__ void copy_cols_scaled(const unsigned char *__restrict__ s,
-                                                 unsigned char *__restrict__ d, uint64_t lds_b,
+__device__ __forceinline__ void copy_cols_scaled(const gbyte *__restrict__ s,
+                                                 gbyte *__restrict__ d, uint64_t lds_b,
                                                  uint64_t ldd_b, uint32_t rows, uint32_t ncols,
                                                  const request &rq) {
     using T = typename bits_t<ES>::type;
     constexpr int U = 4;
     const uint32_t tid = threadIdx.x;
-    const uint32_t txl = rows >= 64 ? 6u : ceil_log2(rows);
-    const uint32_t TX = 1u << txl;
-    const uint32_t NY = (uint32_t)THREADS >> txl;
-    const uint32_t tx = tid & (TX - 1);
-    const uint32_t ty = tid >> txl;
-    for (uint32_t cb = ty; cb < ncols; cb += NY * U) {
-        for (uint32_t v = tx; v < rows; v += TX) {
-            T a[U], b[U];
+    const uint32_t slots = ncols * rows;
+    const fastdiv per_col(rows);
+    for (uint32_t base = tid; base < slots; base += (uint32_t)THREADS * U) {
+        T a[U], b[U];
+        uint64_t dof[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) {
-                    a[u] = *(const T *)(s + c * lds_b + (uint64_t)v * ES);
-                    b[u] = *(const T *)(d + c * ldd_b + (uint64_t)v * ES);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t c = cb + (uint32_t)u * NY;
-                if (c < ncols) *(T *)(d + c * ldd_b + (uint64_t)v * ES) = scale_bits(a[u], b[u], rq);
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = base + (uint32_t)u * THREADS;
+            const uint32_t c = per_col.div(k);
+            const uint32_t v = k - c * rows;
+            dof[u] = c * ldd_b + (uint64_t)v * ES;
+            if (k < slots) {
+                a[u] = *(const GLOBAL T *)(s + c * lds_b + (uint64_t)v * ES);
+                b[u] = *(const GLOBAL T *)(d + dof[u]);
             }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + (uint32_t)u * THREADS < slots) *(GLOBAL T *)(d + dof[u]) = scale_bits(a[u], b[u], rq);
     }
 }
 
-// Transposing copy through an LDS tile: destination(i, j) = op(source(j, i)).
-// Source element (i, j) (final orientation) is at s + (i*lds + j)*ES, i.e.
-// storage rows run along final columns and are read contiguously; the
-// destination is written contiguously along final rows.
+// Element e of a 16-byte vector holding 16/ES elements of ES bytes.
+template <int ES>
+__device__ __forceinline__ typename bits_t<ES>::type vec_elem(const u32x4 &v, int e) {
+    if constexpr (ES == 4)
+        return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+    else if constexpr (ES == 8)
+        return e == 0 ? u32x2{v.x, v.y} : u32x2{v.z, v.w};
+    else
+        return v;
+}
+template <int ES>
+__device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::type x) {
+    if constexpr (ES == 4) {
+        if (e == 0) v.x = x;
+        else if (e == 1) v.y = x;
+        else if (e == 2) v.z = x;
+        else v.w = x;
+    } else if constexpr (ES == 8) {
+        if (e == 0) {
+            v.x = x.x;
+            v.y = x.y;
+        } else {
+            v.z = x.x;
+            v.w = x.y;
+        }
+    } else {
+        v = x;
+    }
+}
+
+constexpr uint32_t kTransRows = 64;  // final rows (storage columns) per transpose tile
+constexpr uint32_t kSmemBytes = kTransRows * (256 + 16);
+
+// Transposing copy: destination(i, j) = op(source(j, i)) for final rows
+// i < rows and final columns j < ncols. Source element (i, j) lies at
+// s + (i*lds + j)*ES: storage rows run along final columns. A tile is
+// TR = 256/ES final columns (a 256-byte stretch of each storage column) by 64
+// final rows. Both phases move 16-byte vectors (dword-aligned addresses), so
+// every 16 lanes of a wave read or write one whole 256-byte segment; LDS holds
+// the tile in storage order, rows padded by one element against bank
+// conflicts. Vectors that would run past the piece move element by element.
 template <int THREADS, bool NT, int ES>
-__device__ __forceinline__ void copy_transposed(const unsigned char *__restrict__ s,
-                                                unsigned char *__restrict__ d, uint64_t lds_b,
+__device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
+                                                gbyte *__restrict__ d, uint64_t lds_b,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
                                                 const request &rq, unsigned char *smem) {
     using T = typename bits_t<ES>::type;
-    T(*tile)[kTileDim + 1] = reinterpret_cast<T(*)[kTileDim + 1]>(smem);
-    const uint32_t x = threadIdx.x & (kTileDim - 1);
-    const uint32_t y = threadIdx.x / kTileDim;
-    constexpr uint32_t YS = THREADS / kTileDim;
+    constexpr uint32_t VE = 16 / ES;             // elements per vector
+    constexpr uint32_t TR = 256 / ES;            // final columns per tile
+    constexpr uint32_t TC = kTransRows;          // final rows per tile
+    constexpr uint32_t LDW = TR + 1;             // LDS row length in elements
+    constexpr uint32_t W = TC * ES / 16;         // vectors per final column of a tile
+    constexpr uint32_t WL = ES == 4 ? 4 : ES == 8 ? 5 : 6;  // log2(W)
+    constexpr uint32_t PER = TC * 16 / THREADS;  // vectors per thread per phase
+    T *tile = reinterpret_cast<T *>(smem);       // tile[c * LDW + r]
     const bool conj = (rq.flags & REQ_CONJ) != 0;
     const bool scale = (rq.flags & REQ_SCALE) != 0;
-    for (uint32_t jc = 0; jc < ncols; jc += kTileDim) {
-        for (uint32_t ic = 0; ic < rows; ic += kTileDim) {
+    for (uint32_t jc = 0; jc < ncols; jc += TR) {
+        for (uint32_t ic = 0; ic < rows; ic += TC) {
 #pragma unroll
-            for (uint32_t k = 0; k < kTileDim; k += YS) {
-                const uint32_t i = ic + y + k, j = jc + x;
-                if (i < rows && j < ncols) tile[y + k][x] = ld<NT>((const T *)(s + i * lds_b + (uint64_t)j * ES));
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t q = threadIdx.x + k * THREADS;
+                const uint32_t c = q >> 4, r0 = (q & 15) * VE;
+                const uint32_t i = ic + c, j0 = jc + r0;
+                if (i < rows && j0 < ncols) {
+                    const gbyte *p = s + i * lds_b + (uint64_t)j0 * ES;
+                    T *t = tile + c * LDW + r0;
+                    if (j0 + VE <= ncols) {
+                        const u32x4 v = ld_a4<NT>(p);
+#pragma unroll
+                        for (uint32_t e = 0; e < VE; ++e) t[e] = vec_elem<ES>(v, e);
+                    } else {
+                        for (uint32_t e = 0; e < ncols - j0; ++e) t[e] = *(const GLOBAL T *)(p + e * ES);
+                    }
+                }
             }
             __syncthreads();
 #pragma unroll
-            for (uint32_t k = 0; k < kTileDim; k += YS) {
-                const uint32_t i = ic + x, j = jc + y + k;
-                if (i < rows && j < ncols) {
-                    T v = tile[x][y + k];
-                    if (conj) v = conj_bits(v);
-                    T *dp = (T *)(d + j * ldd_b + (uint64_t)i * ES);
-                    if (scale) v = scale_bits(v, *dp, rq);
-                    st<NT>(dp, v);
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t q = threadIdx.x + k * THREADS;
+                const uint32_t r = q >> WL, w = q & (W - 1);
+                const uint32_t j = jc + r, i0 = ic + w * VE;
+                if (j < ncols && i0 < rows) {
+                    gbyte *p = d + j * ldd_b + (uint64_t)i0 * ES;
+                    const T *t = tile + (w * VE) * LDW + r;
+                    if (i0 + VE <= rows) {
+                        u32x4 v, old;
+                        if (scale) old = ld_a4<false>(p);
+#pragma unroll
+                        for (uint32_t e = 0; e < VE; ++e) {
+                            T x = t[e * LDW];
+                            if (conj) x = conj_bits(x);
+                            if (scale) x = scale_bits(x, vec_elem<ES>(old, e), rq);
+                            vec_set<ES>(v, e, x);
+                        }
+                        st_a4<NT>(p, v);
+                    } else {
+                        for (uint32_t e = 0; e < rows - i0; ++e) {
+                            T x = t[e * LDW];
+                            if (conj) x = conj_bits(x);
+                            GLOBAL T *dp = (GLOBAL T *)(p + e * ES);
+                            if (scale) x = scale_bits(x, *dp, rq);
+                            *dp = x;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -245,9 +413,9 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     const uint32_t ncols = c1 - c0;
     const uint64_t lds_b = (uint64_t)pc.lds * es;
     const uint64_t ldd_b = (uint64_t)pc.ldd * es;
-    unsigned char *d = (unsigned char *)pc.dst + (uint64_t)c0 * ldd_b;
+    gbyte *d = (gbyte *)pc.dst + (uint64_t)c0 * ldd_b;
     if (rq.flags & REQ_TRANS) {
-        const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * es;
+        const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * es;
         if (es == 4)
             copy_transposed<THREADS, NT, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else if (es == 8)
@@ -256,7 +424,7 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
             copy_transposed<THREADS, NT, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         return;
     }
-    const unsigned char *s = (const unsigned char *)pc.src + (uint64_t)c0 * lds_b;
+    const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * lds_b;
     if (rq.flags & REQ_SCALE) {
         if (es == 4)
             copy_cols_scaled<THREADS, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
@@ -270,15 +438,14 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     const uint64_t al = (uint64_t)s | (uint64_t)d | colb | (ncols > 1 ? (lds_b | ldd_b) : 0);
     if ((al & 15) == 0)
         copy_cols<THREADS, UB, NT, 16>(s, d, lds_b, ldd_b, colb >> 4, ncols);
-    else if ((al & 7) == 0)
-        copy_cols<THREADS, UB, NT, 8>(s, d, lds_b, ldd_b, colb >> 3, ncols);
     else
-        copy_cols<THREADS, UB, NT, 4>(s, d, lds_b, ldd_b, colb >> 2, ncols);
+        copy_cols_dw16<THREADS, UB, NT>(s, d, lds_b, ldd_b, colb, ncols);
 }
 
 // Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
 // stream (pieces concatenated in table order, each column-major). It moves
-// every column (every 32-column group for transposed pieces) whose first
+// every column (every transpose-tile group of 256/sizeof(T) columns for
+// transposed pieces) whose first
 // byte lies in that range, so work items partition the columns exactly.
 template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
@@ -291,7 +458,7 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
         const piece pc = a.pieces[p];
         const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
         const uint64_t colb = (uint64_t)pc.rows * rq.esize;
-        const uint32_t q = (rq.flags & REQ_TRANS) ? (uint32_t)kTileDim : 1u;
+        const uint32_t q = (rq.flags & REQ_TRANS) ? 256u / rq.esize : 1u;
         const uint64_t qb = colb * q;
         uint64_t c0 = 0;
         if (lo > pstart) c0 = ((lo - pstart + qb - 1) / qb) * q;
@@ -318,7 +485,7 @@ __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
 // the chip pulls work indices from a device counter until all are taken.
 template <bool PERSIST, int THREADS, int UB, bool NT>
 __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[kTileDim * (kTileDim + 1) * 16];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kSmemBytes];
     if (!PERSIST) {
         process_work<THREADS, UB, NT>(a, blockIdx.x, smem);
         return;

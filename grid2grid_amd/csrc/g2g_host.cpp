@@ -497,6 +497,7 @@ int ensure_staged(g2g_plan p, hipStream_t s) {
                     pc.cols = 1;
                     len = cnt;
                 }
+                pc.vmul = div_magic((pc.rows * (uint32_t)es + 15) / 16);
                 seg.push_back(pc);
                 cnt -= len;
                 so += len;

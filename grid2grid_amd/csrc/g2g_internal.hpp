@@ -43,8 +43,22 @@ struct piece {
     uint32_t rows;
     uint32_t cols;
     uint32_t req;
-    uint32_t pad;
+    uint32_t vmul;   // division magic for the 16-byte vectors per column, ceil(rows*esize/16)
 };
+
+// Magic multiplier of unsigned 32-bit division by d (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", 1994, fig. 4.1):
+// q = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0), t = mulhi(magic, n),
+// l = ceil(log2(d)).
+__host__ __device__ inline uint32_t div_ceil_log2(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && ((uint64_t)1 << l) < d) ++l;
+    return l;
+}
+__host__ __device__ inline uint32_t div_magic(uint32_t d) {
+    const uint32_t l = div_ceil_log2(d);
+    return (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+}
 static_assert(sizeof(piece) == 40, "piece layout");
 
 // Device block table entry (one per grid cell).

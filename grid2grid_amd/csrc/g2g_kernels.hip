@@ -178,86 +178,81 @@ __device__ __forceinline__ void st_a4(gbyte *p, u32x4 v) {
         *q = w;
 }
 
-// Column-contiguous copy of ncols columns of vpc V-byte vectors each. The
-// ncols*vpc vector slots are numbered column by column and dealt to the
-// threads in order, so consecutive lanes read consecutive vectors of a column
-// (a wave spans several short columns, or one stretch of a long one) and every
-// thread keeps U = UB/V vectors (UB bytes) in flight whatever the piece shape.
-template <int THREADS, int UB, bool NT, int V>
-__device__ __forceinline__ void copy_cols(const gbyte *__restrict__ s,
-                                          gbyte *__restrict__ d, uint64_t lds_b,
-                                          uint64_t ldd_b, uint32_t vpc, uint32_t ncols) {
-    using T = typename bits_t<V>::type;
-    constexpr int U = UB / V;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t slots = ncols * vpc;
-    const fastdiv per_col(vpc);
-    for (uint32_t base = tid; base < slots; base += (uint32_t)THREADS * U) {
-        T r[U];
-        uint64_t so[U], dof[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t k = base + (uint32_t)u * THREADS;
-            const uint32_t c = per_col.div(k);
-            const uint32_t v = k - c * vpc;
-            so[u] = c * lds_b + (uint64_t)v * V;
-            dof[u] = c * ldd_b + (uint64_t)v * V;
-            if (k < slots) r[u] = ld<NT>((const GLOBAL T *)(s + so[u]));
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (base + (uint32_t)u * THREADS < slots) st<NT>((GLOBAL T *)(d + dof[u]), r[u]);
-        }
-    }
+// Column runs queued for one flat copy: `cols` columns of colb bytes starting
+// at s (source, leading dimension lds_b bytes) and d (destination, ldd_b).
+// Each column is cut into vpc = ceil(colb/16) 16-byte slots, the last one
+// partial when colb is not a multiple of 16; slot0 numbers the segment's first
+// slot in the queue.
+struct seg_entry {
+    uint64_t s, d;
+    uint64_t lds_b, ldd_b;
+    uint32_t colb;
+    uint32_t vpc;
+    uint32_t slot0;
+    uint32_t vmul;  // division magic for vpc
+};
+constexpr int kMaxSeg = 64;
+
+__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t mul) {
+    const uint32_t l = ceil_log2(d);
+    const uint32_t t = __umulhi(mul, n);
+    return l ? (t + ((n - t) >> 1)) >> (l - 1) : n;
 }
 
-// The same copy for columns that are only 4-byte aligned (padded leading
-// dimensions, blocks at arbitrary element offsets): 16-byte accesses at
-// dword-aligned addresses, and the last 1-3 dwords of a column whose byte
-// length is not a multiple of 16 moved one by one.
+// Copy every slot of the queued segments. Slots are dealt to the threads in
+// order, so consecutive lanes move consecutive 16-byte vectors of a column (a
+// wave spans several short columns, or one stretch of a long one, and several
+// pieces) and every thread keeps UB bytes in flight whatever the piece shapes.
+// Addresses need only be dword-aligned.
 template <int THREADS, int UB, bool NT>
-__device__ __forceinline__ void copy_cols_dw16(const gbyte *__restrict__ s,
-                                               gbyte *__restrict__ d, uint64_t lds_b,
-                                               uint64_t ldd_b, uint32_t colb, uint32_t ncols) {
+__device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total) {
     constexpr int U = UB / 16;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t vpc = (colb + 15) >> 4;
-    const uint32_t tail = (colb & 15) >> 2;  // dwords of a partial last vector (0: none)
-    const uint32_t slots = ncols * vpc;
-    const fastdiv per_col(vpc);
-    for (uint32_t base = tid; base < slots; base += (uint32_t)THREADS * U) {
+    // the segment of this thread's current slot, kept in registers (a
+    // thread's slots only increase, so it moves forward and rarely)
+    uint32_t j = 0;
+    seg_entry e = segs[0];
+    uint32_t next0 = segs[1].slot0;
+    for (uint32_t base = threadIdx.x; base < total; base += (uint32_t)THREADS * U) {
         u32x4 r[U];
-        uint64_t so[U], dof[U];
-        bool part[U];
+        gbyte *dp[U];
+        uint32_t tail[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t k = base + (uint32_t)u * THREADS;
-            const uint32_t c = per_col.div(k);
-            const uint32_t v = k - c * vpc;
-            so[u] = c * lds_b + (uint64_t)v * 16;
-            dof[u] = c * ldd_b + (uint64_t)v * 16;
-            part[u] = tail != 0 && v + 1 == vpc;
-            if (k < slots) {
-                if (!part[u]) {
-                    r[u] = ld_a4<NT>(s + so[u]);
+            tail[u] = 4;
+            if (k < total) {
+                while (next0 <= k) {
+                    ++j;
+                    e = segs[j];
+                    next0 = segs[j + 1].slot0;
+                }
+                const uint32_t rel = k - e.slot0;
+                const uint32_t c = magic_div(rel, e.vpc, e.vmul);
+                const uint32_t v = rel - c * e.vpc;
+                const gbyte *sp = (const gbyte *)e.s + c * e.lds_b + (uint64_t)v * 16;
+                dp[u] = (gbyte *)e.d + c * e.ldd_b + (uint64_t)v * 16;
+                // dwords in this slot: 4, or 1-3 for the last slot of a ragged column
+                tail[u] = (v + 1 == e.vpc && (e.colb & 15)) ? (e.colb & 15) >> 2 : 4;
+                if (tail[u] == 4) {
+                    r[u] = ld_a4<NT>(sp);
                 } else {
-                    const GLOBAL uint32_t *p = (const GLOBAL uint32_t *)(s + so[u]);
-                    r[u].x = p[0];
-                    r[u].y = tail > 1 ? p[1] : 0u;
-                    r[u].z = tail > 2 ? p[2] : 0u;
+                    const GLOBAL uint32_t *q = (const GLOBAL uint32_t *)sp;
+                    r[u].x = q[0];
+                    r[u].y = tail[u] > 1 ? q[1] : 0u;
+                    r[u].z = tail[u] > 2 ? q[2] : 0u;
                 }
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (base + (uint32_t)u * THREADS >= slots) continue;
-            if (!part[u]) {
-                st_a4<NT>(d + dof[u], r[u]);
+            if (base + (uint32_t)u * THREADS >= total) continue;
+            if (tail[u] == 4) {
+                st_a4<NT>(dp[u], r[u]);
             } else {
-                GLOBAL uint32_t *p = (GLOBAL uint32_t *)(d + dof[u]);
-                p[0] = r[u].x;
-                if (tail > 1) p[1] = r[u].y;
-                if (tail > 2) p[2] = r[u].z;
+                GLOBAL uint32_t *q = (GLOBAL uint32_t *)dp[u];
+                q[0] = r[u].x;
+                if (tail[u] > 1) q[1] = r[u].y;
+                if (tail[u] > 2) q[2] = r[u].z;
             }
         }
     }
@@ -406,7 +401,8 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
     }
 }
 
-template <int THREADS, int UB, bool NT>
+// Transposed or scaled pieces: moved by the whole workgroup right away.
+template <int THREADS, bool NT>
 __device__ __forceinline__ void process_piece(const piece &pc, const request &rq, uint32_t c0,
                                               uint32_t c1, unsigned char *smem) {
     const uint32_t es = rq.esize;
@@ -425,60 +421,94 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
         return;
     }
     const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * lds_b;
-    if (rq.flags & REQ_SCALE) {
-        if (es == 4)
-            copy_cols_scaled<THREADS, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
-        else if (es == 8)
-            copy_cols_scaled<THREADS, 8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
-        else
-            copy_cols_scaled<THREADS, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
-        return;
-    }
-    const uint32_t colb = pc.rows * es;
-    const uint64_t al = (uint64_t)s | (uint64_t)d | colb | (ncols > 1 ? (lds_b | ldd_b) : 0);
-    if ((al & 15) == 0)
-        copy_cols<THREADS, UB, NT, 16>(s, d, lds_b, ldd_b, colb >> 4, ncols);
+    if (es == 4)
+        copy_cols_scaled<THREADS, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
+    else if (es == 8)
+        copy_cols_scaled<THREADS, 8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
     else
-        copy_cols_dw16<THREADS, UB, NT>(s, d, lds_b, ldd_b, colb, ncols);
+        copy_cols_scaled<THREADS, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
 }
 
 // Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
 // stream (pieces concatenated in table order, each column-major). It moves
-// every column (every transpose-tile group of 256/sizeof(T) columns for
-// transposed pieces) whose first
-// byte lies in that range, so work items partition the columns exactly.
+// every column (every group of 256/sizeof(T) columns for transposed pieces)
+// whose first byte lies in that range, so work items partition the columns
+// exactly. Untransposed, unscaled column runs are queued (up to kMaxSeg
+// pieces) and moved together by flat_copy; transposed and scaled pieces are
+// moved as they come.
 template <int THREADS, int UB, bool NT>
-__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t,
-                                             unsigned char *smem) {
+__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, unsigned char *smem,
+                                             seg_entry *segs) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
     const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
-    for (uint32_t p = a.first_piece[t];; ++p) {
-        const uint64_t pstart = p ? a.pend[p - 1] : 0;
-        if (pstart >= hi) break;
-        const piece pc = a.pieces[p];
-        const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
-        const uint64_t colb = (uint64_t)pc.rows * rq.esize;
-        const uint32_t q = (rq.flags & REQ_TRANS) ? 256u / rq.esize : 1u;
-        const uint64_t qb = colb * q;
-        uint64_t c0 = 0;
-        if (lo > pstart) c0 = ((lo - pstart + qb - 1) / qb) * q;
-        uint64_t c1 = ((hi - pstart + qb - 1) / qb) * q;
-        if (c1 > pc.cols) c1 = pc.cols;
-        if (c0 < c1) process_piece<THREADS, UB, NT>(pc, rq, (uint32_t)c0, (uint32_t)c1, smem);
+    uint32_t p = a.first_piece[t];
+    uint64_t pstart = p ? a.pend[p - 1] : 0;
+    bool more = true;
+    while (more) {
+        uint32_t nseg = 0, total = 0;
+        for (;;) {
+            if (pstart >= hi) {
+                more = false;
+                break;
+            }
+            if (nseg == kMaxSeg) break;
+            const piece pc = a.pieces[p];
+            const uint64_t pend = a.pend[p];
+            const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
+            const uint32_t es = rq.esize;
+            const uint32_t colb = pc.rows * es;
+            uint32_t c0 = 0, c1 = pc.cols;
+            if (pstart < lo || pend > hi) {  // piece shared with neighbouring work items
+                const uint64_t q = (rq.flags & REQ_TRANS) ? 256u / es : 1u;
+                const uint64_t qb = (uint64_t)colb * q;
+                const uint64_t a0 = lo > pstart ? ((lo - pstart + qb - 1) / qb) * q : 0;
+                const uint64_t a1 = ((hi - pstart + qb - 1) / qb) * q;
+                c0 = (uint32_t)min(a0, (uint64_t)pc.cols);
+                c1 = (uint32_t)min(a1, (uint64_t)pc.cols);
+            }
+            ++p;
+            pstart = pend;
+            if (c0 >= c1) continue;
+            if (rq.flags & (REQ_TRANS | REQ_SCALE)) {
+                process_piece<THREADS, NT>(pc, rq, c0, c1, smem);
+                continue;
+            }
+            const uint64_t lds_b = (uint64_t)pc.lds * es, ldd_b = (uint64_t)pc.ldd * es;
+            const uint32_t vpc = (colb + 15) >> 4;
+            if (threadIdx.x == 0) {
+                seg_entry &e = segs[nseg];
+                e.s = pc.src + (uint64_t)c0 * lds_b;
+                e.d = pc.dst + (uint64_t)c0 * ldd_b;
+                e.lds_b = lds_b;
+                e.ldd_b = ldd_b;
+                e.colb = colb;
+                e.vpc = vpc;
+                e.slot0 = total;
+                e.vmul = pc.vmul;
+            }
+            total += (c1 - c0) * vpc;
+            ++nseg;
+        }
+        if (nseg) {
+            if (threadIdx.x == 0) segs[nseg].slot0 = total;  // sentinel
+            __syncthreads();
+            flat_copy<THREADS, UB, NT>(segs, total);
+            __syncthreads();
+        }
     }
 }
 
 // Global work index w -> (transform, local work item) -> columns to move.
 template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
-                                             unsigned char *smem) {
+                                             unsigned char *smem, seg_entry *segs) {
     if (!a.descs) {
-        process_tile<THREADS, UB, NT>(a.one, w, smem);
+        process_tile<THREADS, UB, NT>(a.one, w, smem, segs);
         return;
     }
     const uint32_t k = a.tile_owner[w];
     const work_desc d = a.descs[k];
-    process_tile<THREADS, UB, NT>(d, w - d.tile_base, smem);
+    process_tile<THREADS, UB, NT>(d, w - d.tile_base, smem, segs);
 }
 
 // Direct mode: one workgroup per work index. Persistent mode: a grid sized to
@@ -486,8 +516,9 @@ __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
 template <bool PERSIST, int THREADS, int UB, bool NT>
 __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[kSmemBytes];
+    __shared__ seg_entry segs[kMaxSeg + 1];
     if (!PERSIST) {
-        process_work<THREADS, UB, NT>(a, blockIdx.x, smem);
+        process_work<THREADS, UB, NT>(a, blockIdx.x, smem, segs);
         return;
     }
     __shared__ uint32_t s_t;
@@ -497,7 +528,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
         if (t >= a.total_tiles) break;
-        process_work<THREADS, UB, NT>(a, t, smem);
+        process_work<THREADS, UB, NT>(a, t, smem, segs);
     }
 }
 
@@ -586,7 +617,7 @@ __global__ void k_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
     pc.rows = rows;
     pc.cols = cols;
     pc.req = 0;
-    pc.pad = 0;
+    pc.vmul = div_magic((rows * es + 15) / 16);
     pieces[p] = pc;
     bytes[p] = (uint64_t)rows * cols * es;
     ranks[p] = ((uint32_t)I.owner << 16) | (uint32_t)F.owner;

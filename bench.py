@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline leg")
     ap.add_argument("--batch-requests", type=int, default=2048)
+    ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
+    ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
+    ap.add_argument("--only-batched", action="store_true", help="run only the batched leg")
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)
@@ -183,15 +186,21 @@ def verify_requests(g2g, rs):
     return bad
 
 
-def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev):
+def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, tile_bytes=0):
     """Config 4: n_req mixed transforms in one persistent launch (hipGraph
     replay) versus plain per-request launches (the staged form per request)
     and fused per-request launches."""
     t0 = time.time()
-    rs = make_requests(torch, g2g, workloads.cfg4(n=n_req), 40000, dev)
+    if tile_bytes:
+        g2g.set_tuning(tile_bytes, -1)
+    rs = make_requests(torch, g2g, workloads.cfg4(n=n_req, p_trans=p_trans), 40000, dev)
+    g2g.set_tuning(0, -1)
     nbytes = sum(r["bytes"] for r in rs)
     log(f"cfg4: {n_req} requests, {nbytes / 1e9:.2f} GB moved per pass, planned in {time.time() - t0:.1f}s")
     batch = g2g.Batch([r["plan"] for r in rs], stream=stream)
+    t0 = time.time()
+    btune = batch.autotune(stream, reps=2)
+    log(f"batch autotune ({time.time() - t0:.1f}s): {btune}")
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         ms_b, _ = time_steps(torch, lambda: batch.execute_graph(s), steps, 2, s)
@@ -218,6 +227,7 @@ def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev):
         "fused_per_request": {"ms": round(ms_f, 3), "GB/s": round(nbytes / ms_f / 1e6, 1), "bit_exact": bad_f == 0},
         "speedup_vs_plain": round(ms_p / ms_b, 2),
         "speedup_vs_fused_per_request": round(ms_f / ms_b, 2),
+        "batch_tuning": {k: btune[k] for k in ("tile_bytes", "threads", "unroll_bytes", "nontemporal")},
     }
     for r in rs:
         r["plan"].close()
@@ -326,6 +336,12 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
 
+    if args.only_batched:
+        out = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev, args.batch_ptrans,
+                          args.batch_tile_bytes)
+        print(json.dumps({"batched": out}), flush=True)
+        return
+
     init, fin, dtype, trans = build_workload(args.workload)
     es = g2g.DTYPE_SIZE[dtype]
     log(f"workload {args.workload}: {WORKLOAD_DESC.get(args.workload, args.workload)}")
@@ -411,7 +427,8 @@ def main():
     torch.cuda.empty_cache()
 
     if not args.no_extras:
-        result["batched"] = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev)
+        result["batched"] = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev,
+                                        args.batch_ptrans, args.batch_tile_bytes)
         log(f"batched: {result['batched']}")
         result["served"] = served_leg(torch, g2g, workloads, 512, 6, args.serve_requests, args.qps,
                                       int(args.budget_gb * 1e9), dev)

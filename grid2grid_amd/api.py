@@ -160,6 +160,11 @@ class Batch:
         check(_lib.load().g2g_batch_bytes(self.handle, ctypes.byref(v)))
         return v.value
 
+    def autotune(self, stream=None, reps=2):
+        t = _lib.G2GTuning()
+        check(_lib.load().g2g_batch_autotune(self.handle, _stream(stream), reps, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().g2g_batch_destroy(self.handle)

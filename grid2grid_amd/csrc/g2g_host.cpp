@@ -539,6 +539,7 @@ int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
     a.queue = p->queue.p;
     launch_cfg c = t.cfg;
     if (g_persistent.load() >= 0) c.persistent = g_persistent.load() != 0;
+    c.features = std::max(request_features(kr.r[0]), request_features(kr.r[1]));
     G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
 }
@@ -700,6 +701,7 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
                         c.unroll_bytes = ub;
                         c.nontemporal = nt != 0;
                         c.persistent = pe != 0;
+                        c.features = request_features(kr.r[0]);
                         p->fused.cfg = c;
                         gather_args a;
                         std::memset(&a, 0, sizeof a);
@@ -840,17 +842,25 @@ int g2g_plan_send_buffer(g2g_plan p, int rank, const void **data, int64_t *elems
 
 struct g2g_batch_s {
     std::vector<g2g_plan> plans;
+    std::vector<work_desc> host_descs;  // requests as the batch runs them
     dbuf<work_desc> descs;
+    dbuf<uint32_t> first;               // per plan: first piece of each batch work item
     dbuf<uint32_t> owner;
     dbuf<uint32_t> queue;
     uint32_t total_tiles = 0;
     uint64_t total_bytes = 0;
+    uint64_t tile_bytes = 0;
+    launch_cfg cfg;
+    double tuned_ms = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    ~g2g_batch_s() {
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
     }
+    ~g2g_batch_s() { drop_graph(); }
 };
 
 namespace {
@@ -862,11 +872,45 @@ int batch_launch(g2g_batch b, hipStream_t s) {
     a.tile_owner = b->owner.p;
     a.total_tiles = b->total_tiles;
     a.queue = b->queue.p;
-    launch_cfg c;
-    c.persistent = true;
+    launch_cfg c = b->cfg;
+    c.features = FEAT_COPY;
+    for (const work_desc &d : b->host_descs)
+        c.features = std::max(c.features, std::max(request_features(d.kr.r[0]), request_features(d.kr.r[1])));
     G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
 }
+
+// Work items of a batch: every plan's piece stream cut at the batch's own
+// work-item size (a batch of small transforms wants larger items than each
+// transform run alone), numbered plan after plan.
+int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
+    b->drop_graph();
+    b->tile_bytes = tile_bytes;
+    std::vector<uint64_t> off(b->plans.size());
+    uint64_t tiles = 0;
+    for (size_t k = 0; k < b->plans.size(); ++k) {
+        off[k] = tiles;
+        tiles += (b->plans[k]->fused.total + tile_bytes - 1) / tile_bytes;
+    }
+    if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
+    b->total_tiles = (uint32_t)tiles;
+    G2G_TRY(b->first.alloc(tiles));
+    G2G_TRY(b->owner.alloc(tiles));
+    for (size_t k = 0; k < b->plans.size(); ++k) {
+        const stage_table &t = b->plans[k]->fused;
+        work_desc &d = b->host_descs[k];
+        d.first_piece = b->first.p + off[k];
+        d.tile_bytes = tile_bytes;
+        d.n_tiles = (uint32_t)((t.total + tile_bytes - 1) / tile_bytes);
+        d.tile_base = (uint32_t)off[k];
+        G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, b->first.p + off[k], s));
+    }
+    G2G_TRY(b->descs.upload(b->host_descs.data(), b->host_descs.size(), s));
+    G2G_HIP(launch_tile_owner(b->descs.p, (int32_t)b->host_descs.size(), b->owner.p, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
 
 }  // namespace
 
@@ -922,14 +966,13 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
         g2g_batch &b;
         ~guard() { delete b; }
     } g{b};
-    std::vector<work_desc> descs;
-    G2G_TRY(internal_descs(n, plans, alphas, betas, descs, b->total_tiles, b->total_bytes));
+    uint32_t tiles = 0;
+    G2G_TRY(internal_descs(n, plans, alphas, betas, b->host_descs, tiles, b->total_bytes));
     b->plans.assign(plans, plans + n);
-    G2G_TRY(b->descs.upload(descs.data(), descs.size(), s));
-    G2G_TRY(b->owner.alloc(b->total_tiles));
     G2G_TRY(b->queue.alloc(1));
-    G2G_HIP(launch_tile_owner(b->descs.p, n, b->owner.p, s));
-    G2G_HIP(hipStreamSynchronize(s));
+    b->cfg.persistent = true;
+    const int64_t forced = g_tile_bytes.load();
+    G2G_TRY(batch_retile(b, forced > 0 ? (uint64_t)forced : (uint64_t)131072, s));
     *out = b;
     b = nullptr;
     return G2G_OK;
@@ -958,6 +1001,59 @@ int g2g_batch_execute_graph(g2g_batch b, void *stream) {
         G2G_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
     }
     G2G_HIP(hipGraphLaunch(b->exec, s));
+    return G2G_OK;
+}
+
+int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
+    if (!b) return fail(G2G_ERR_INVALID, "batch is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    reps = std::max(1, reps);
+    hipEvent_t e0, e1;
+    G2G_HIP(hipEventCreate(&e0));
+    G2G_HIP(hipEventCreate(&e1));
+    double best_ms = 1e30;
+    uint64_t best_tb = b->tile_bytes;
+    launch_cfg best_cfg = b->cfg;
+    int rc = G2G_OK;
+    for (uint64_t tb = 32768; tb <= (1u << 20) && rc == G2G_OK; tb *= 2) {
+        if ((rc = batch_retile(b, tb, s)) != G2G_OK) break;
+        for (int th : {256, 512})
+            for (int ub : {64, 128})
+                for (int nt = 0; nt < 2 && rc == G2G_OK; ++nt) {
+                    b->cfg.threads = th;
+                    b->cfg.unroll_bytes = ub;
+                    b->cfg.nontemporal = nt != 0;
+                    b->cfg.persistent = true;
+                    rc = batch_launch(b, s);  // warm-up
+                    hipError_t he = rc == G2G_OK ? hipEventRecord(e0, s) : hipErrorUnknown;
+                    for (int r = 0; r < reps && he == hipSuccess && rc == G2G_OK; ++r) rc = batch_launch(b, s);
+                    if (he == hipSuccess && rc == G2G_OK) he = hipEventRecord(e1, s);
+                    if (he == hipSuccess && rc == G2G_OK) he = hipEventSynchronize(e1);
+                    float ms = 0;
+                    if (he == hipSuccess && rc == G2G_OK) he = hipEventElapsedTime(&ms, e0, e1);
+                    if (rc == G2G_OK && he != hipSuccess)
+                        rc = fail(G2G_ERR_HIP, "batch autotune: %s", hipGetErrorString(he));
+                    if (rc == G2G_OK && ms / reps < best_ms) {
+                        best_ms = ms / reps;
+                        best_tb = tb;
+                        best_cfg = b->cfg;
+                    }
+                }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != G2G_OK) return rc;
+    G2G_TRY(batch_retile(b, best_tb, s));
+    b->cfg = best_cfg;
+    b->tuned_ms = best_ms;
+    if (best) {
+        best->tile_bytes = (int64_t)b->tile_bytes;
+        best->threads = b->cfg.threads;
+        best->unroll_bytes = b->cfg.unroll_bytes;
+        best->nontemporal = b->cfg.nontemporal;
+        best->persistent = 1;
+        best->ms = b->tuned_ms;
+    }
     return G2G_OK;
 }
 

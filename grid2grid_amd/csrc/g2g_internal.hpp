@@ -146,12 +146,27 @@ struct gather_args {
 };
 // Compiled variants of the gather: workgroup size, bytes each thread keeps in
 // flight, non-temporal hint, launch mode.
+// Feature levels: the smallest kernel that covers every request of a launch
+// (fewer paths, fewer registers, more waves per CU).
+enum : int {
+    FEAT_COPY = 0,   // untransposed, unscaled pieces only
+    FEAT_TRANS = 1,  // plus transposed / conjugated pieces
+    FEAT_ALL = 2     // plus scaling
+};
 struct launch_cfg {
     int threads = 256;       // 256 or 512
     int unroll_bytes = 64;   // 64 or 128
     bool nontemporal = false;
     bool persistent = false;
+    int features = FEAT_ALL;
 };
+
+// Feature level a request needs.
+inline int request_features(const request &r) {
+    if (r.flags & REQ_SCALE) return FEAT_ALL;
+    if (r.flags & (REQ_TRANS | REQ_CONJ)) return FEAT_TRANS;
+    return FEAT_COPY;
+}
 // `cus` = compute units of the device (persistent grids fill them).
 hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream);
 

@@ -178,11 +178,16 @@ __device__ __forceinline__ void st_a4(gbyte *p, u32x4 v) {
         *q = w;
 }
 
-// Column runs queued for one flat copy: `cols` columns of colb bytes starting
-// at s (source, leading dimension lds_b bytes) and d (destination, ldd_b).
-// Each column is cut into vpc = ceil(colb/16) 16-byte slots, the last one
-// partial when colb is not a multiple of 16; slot0 numbers the segment's first
-// slot in the queue.
+// Runs queued for one flat copy.
+// SEG_VEC: columns of colb bytes starting at s (source, leading dimension
+//   lds_b bytes) and d (destination, ldd_b), each cut into vpc = ceil(colb/16)
+//   16-byte slots, the last one partial when colb is not a multiple of 16.
+// SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece, one
+//   slot per element (vpc = final rows per final column): destination element
+//   (i, c) at d + c*ldd_b + i*es comes from s + i*lds_b + c*es, its imaginary
+//   sign bit flipped for SEG_CONJ.
+// slot0 numbers the segment's first slot in its queue.
+enum : uint32_t { SEG_VEC = 0, SEG_TRANS = 1, SEG_CONJ = 2 };
 struct seg_entry {
     uint64_t s, d;
     uint64_t lds_b, ldd_b;
@@ -190,6 +195,8 @@ struct seg_entry {
     uint32_t vpc;
     uint32_t slot0;
     uint32_t vmul;  // division magic for vpc
+    uint32_t kind;
+    uint32_t pad;
 };
 constexpr int kMaxSeg = 64;
 
@@ -199,11 +206,11 @@ __device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t m
     return l ? (t + ((n - t) >> 1)) >> (l - 1) : n;
 }
 
-// Copy every slot of the queued segments. Slots are dealt to the threads in
-// order, so consecutive lanes move consecutive 16-byte vectors of a column (a
-// wave spans several short columns, or one stretch of a long one, and several
-// pieces) and every thread keeps UB bytes in flight whatever the piece shapes.
-// Addresses need only be dword-aligned.
+// Copy every slot of the queued SEG_VEC segments. Slots are dealt to the
+// threads in order, so consecutive lanes move consecutive 16-byte vectors of a
+// column (a wave spans several short columns, or one stretch of a long one,
+// and several pieces), and every thread keeps UB bytes in flight whatever the
+// piece shapes. Addresses need only be dword-aligned.
 template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total) {
     constexpr int U = UB / 16;
@@ -255,6 +262,42 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
                 if (tail[u] > 2) q[2] = r[u].z;
             }
         }
+    }
+}
+
+// Copy every element slot of the queued SEG_TRANS / SEG_CONJ segments of
+// ES-byte elements: consecutive lanes write consecutive elements of a
+// destination column (and read a strided row of the source).
+template <int THREADS, bool NT, int ES>
+__device__ __forceinline__ void flat_copy_elems(const seg_entry *segs, uint32_t total) {
+    using T = typename bits_t<ES>::type;
+    constexpr int U = ES == 16 ? 4 : 8;
+    uint32_t j = 0;
+    seg_entry e = segs[0];
+    uint32_t next0 = segs[1].slot0;
+    for (uint32_t base = threadIdx.x; base < total; base += (uint32_t)THREADS * U) {
+        T r[U];
+        gbyte *dp[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = base + (uint32_t)u * THREADS;
+            if (k < total) {
+                while (next0 <= k) {
+                    ++j;
+                    e = segs[j];
+                    next0 = segs[j + 1].slot0;
+                }
+                const uint32_t rel = k - e.slot0;
+                const uint32_t c = magic_div(rel, e.vpc, e.vmul);
+                const uint32_t v = rel - c * e.vpc;
+                dp[u] = (gbyte *)e.d + c * e.ldd_b + (uint64_t)v * ES;
+                r[u] = ld<NT>((const GLOBAL T *)((const gbyte *)e.s + (uint64_t)v * e.lds_b + (uint64_t)c * ES));
+                if (ES > 4 && e.kind == SEG_CONJ) r[u] = conj_bits(r[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + (uint32_t)u * THREADS < total) st<NT>((GLOBAL T *)dp[u], r[u]);
     }
 }
 
@@ -330,7 +373,7 @@ constexpr uint32_t kSmemBytes = kTransRows * (256 + 16);
 // every 16 lanes of a wave read or write one whole 256-byte segment; LDS holds
 // the tile in storage order, rows padded by one element against bank
 // conflicts. Vectors that would run past the piece move element by element.
-template <int THREADS, bool NT, int ES>
+template <int THREADS, bool NT, int ES, bool SCALE>
 __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
                                                 gbyte *__restrict__ d, uint64_t lds_b,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
@@ -345,7 +388,7 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
     constexpr uint32_t PER = TC * 16 / THREADS;  // vectors per thread per phase
     T *tile = reinterpret_cast<T *>(smem);       // tile[c * LDW + r]
     const bool conj = (rq.flags & REQ_CONJ) != 0;
-    const bool scale = (rq.flags & REQ_SCALE) != 0;
+    const bool scale = SCALE && (rq.flags & REQ_SCALE) != 0;
     for (uint32_t jc = 0; jc < ncols; jc += TR) {
         for (uint32_t ic = 0; ic < rows; ic += TC) {
 #pragma unroll
@@ -402,7 +445,7 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
 }
 
 // Transposed or scaled pieces: moved by the whole workgroup right away.
-template <int THREADS, bool NT>
+template <int THREADS, bool NT, int FEAT>
 __device__ __forceinline__ void process_piece(const piece &pc, const request &rq, uint32_t c0,
                                               uint32_t c1, unsigned char *smem) {
     const uint32_t es = rq.esize;
@@ -413,13 +456,14 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     if (rq.flags & REQ_TRANS) {
         const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * es;
         if (es == 4)
-            copy_transposed<THREADS, NT, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 4, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else if (es == 8)
-            copy_transposed<THREADS, NT, 8>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 8, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else
-            copy_transposed<THREADS, NT, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed<THREADS, NT, 16, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         return;
     }
+    if constexpr (FEAT != FEAT_ALL) return;  // the host never sends scaling here
     const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * lds_b;
     if (es == 4)
         copy_cols_scaled<THREADS, 4>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
@@ -434,24 +478,25 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
 // every column (every group of 256/sizeof(T) columns for transposed pieces)
 // whose first byte lies in that range, so work items partition the columns
 // exactly. Untransposed, unscaled column runs are queued (up to kMaxSeg
-// pieces) and moved together by flat_copy; transposed and scaled pieces are
-// moved as they come.
-template <int THREADS, int UB, bool NT>
+// pieces) and moved together by flat_copy; small transposed pieces are queued
+// likewise for flat_copy_elems; large transposed and scaled pieces are moved
+// as they come.
+template <int THREADS, int UB, bool NT, int FEAT>
 __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, unsigned char *smem,
-                                             seg_entry *segs) {
+                                             seg_entry *segs, seg_entry *tsegs) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
     const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
     uint32_t p = a.first_piece[t];
     uint64_t pstart = p ? a.pend[p - 1] : 0;
     bool more = true;
     while (more) {
-        uint32_t nseg = 0, total = 0;
+        uint32_t nseg = 0, total = 0, ntseg = 0, ttotal = 0, tes = 4;
         for (;;) {
             if (pstart >= hi) {
                 more = false;
                 break;
             }
-            if (nseg == kMaxSeg) break;
+            if (nseg == kMaxSeg || ntseg == kMaxSeg) break;
             const piece pc = a.pieces[p];
             const uint64_t pend = a.pend[p];
             const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
@@ -469,12 +514,45 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uns
             ++p;
             pstart = pend;
             if (c0 >= c1) continue;
-            if (rq.flags & (REQ_TRANS | REQ_SCALE)) {
-                process_piece<THREADS, NT>(pc, rq, c0, c1, smem);
-                continue;
-            }
+            const bool trans = (rq.flags & REQ_TRANS) != 0;
             const uint64_t lds_b = (uint64_t)pc.lds * es, ldd_b = (uint64_t)pc.ldd * es;
-            const uint32_t vpc = (colb + 15) >> 4;
+            if constexpr (FEAT == FEAT_COPY) {
+                // the copy-only variant is launched for untransposed, unscaled
+                // requests only; anything else is left alone rather than misread
+                if (rq.flags & (REQ_TRANS | REQ_SCALE)) continue;
+            } else {
+                if (rq.flags & REQ_SCALE) {
+                    process_piece<THREADS, NT, FEAT>(pc, rq, c0, c1, smem);
+                    continue;
+                }
+                if (trans) {
+                    // transposed pieces that would leave an LDS tile mostly
+                    // empty move element by element in their own queue; a
+                    // queue holds one element size
+                    const bool small_t = pc.rows < kTransRows || (c1 - c0) < 256u / es;
+                    if (!small_t || (ntseg && es != tes)) {
+                        process_piece<THREADS, NT, FEAT>(pc, rq, c0, c1, smem);
+                        continue;
+                    }
+                    if (threadIdx.x == 0) {
+                        seg_entry &e = tsegs[ntseg];
+                        e.s = pc.src + (uint64_t)c0 * es;
+                        e.d = pc.dst + (uint64_t)c0 * ldd_b;
+                        e.lds_b = lds_b;
+                        e.ldd_b = ldd_b;
+                        e.colb = colb;
+                        e.vpc = pc.rows;
+                        e.slot0 = ttotal;
+                        e.vmul = div_magic(pc.rows);
+                        e.kind = (rq.flags & REQ_CONJ) ? SEG_CONJ : SEG_TRANS;
+                        e.pad = 0;
+                    }
+                    tes = es;
+                    ttotal += (c1 - c0) * pc.rows;
+                    ++ntseg;
+                    continue;
+                }
+            }
             if (threadIdx.x == 0) {
                 seg_entry &e = segs[nseg];
                 e.s = pc.src + (uint64_t)c0 * lds_b;
@@ -482,43 +560,56 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uns
                 e.lds_b = lds_b;
                 e.ldd_b = ldd_b;
                 e.colb = colb;
-                e.vpc = vpc;
+                e.vpc = (colb + 15) >> 4;
                 e.slot0 = total;
                 e.vmul = pc.vmul;
+                e.kind = SEG_VEC;
+                e.pad = 0;
             }
-            total += (c1 - c0) * vpc;
+            total += (c1 - c0) * ((colb + 15) >> 4);
             ++nseg;
         }
-        if (nseg) {
-            if (threadIdx.x == 0) segs[nseg].slot0 = total;  // sentinel
+        if (nseg | ntseg) {
+            if (threadIdx.x == 0) {  // sentinels
+                segs[nseg].slot0 = total;
+                tsegs[ntseg].slot0 = ttotal;
+            }
             __syncthreads();
-            flat_copy<THREADS, UB, NT>(segs, total);
+            if (nseg) flat_copy<THREADS, UB, NT>(segs, total);
+            if constexpr (FEAT != FEAT_COPY) {
+                if (ntseg) {
+                    if (tes == 4) flat_copy_elems<THREADS, NT, 4>(tsegs, ttotal);
+                    else if (tes == 8) flat_copy_elems<THREADS, NT, 8>(tsegs, ttotal);
+                    else flat_copy_elems<THREADS, NT, 16>(tsegs, ttotal);
+                }
+            }
             __syncthreads();
         }
     }
 }
 
 // Global work index w -> (transform, local work item) -> columns to move.
-template <int THREADS, int UB, bool NT>
+template <int THREADS, int UB, bool NT, int FEAT>
 __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
-                                             unsigned char *smem, seg_entry *segs) {
+                                             unsigned char *smem, seg_entry *segs, seg_entry *tsegs) {
     if (!a.descs) {
-        process_tile<THREADS, UB, NT>(a.one, w, smem, segs);
+        process_tile<THREADS, UB, NT, FEAT>(a.one, w, smem, segs, tsegs);
         return;
     }
     const uint32_t k = a.tile_owner[w];
     const work_desc d = a.descs[k];
-    process_tile<THREADS, UB, NT>(d, w - d.tile_base, smem, segs);
+    process_tile<THREADS, UB, NT, FEAT>(d, w - d.tile_base, smem, segs, tsegs);
 }
 
 // Direct mode: one workgroup per work index. Persistent mode: a grid sized to
 // the chip pulls work indices from a device counter until all are taken.
-template <bool PERSIST, int THREADS, int UB, bool NT>
+template <bool PERSIST, int THREADS, int UB, bool NT, int FEAT>
 __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[kSmemBytes];
     __shared__ seg_entry segs[kMaxSeg + 1];
+    __shared__ seg_entry tsegs[FEAT == FEAT_COPY ? 1 : kMaxSeg + 1];
     if (!PERSIST) {
-        process_work<THREADS, UB, NT>(a, blockIdx.x, smem, segs);
+        process_work<THREADS, UB, NT, FEAT>(a, blockIdx.x, smem, segs, tsegs);
         return;
     }
     __shared__ uint32_t s_t;
@@ -528,7 +619,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
         if (t >= a.total_tiles) break;
-        process_work<THREADS, UB, NT>(a, t, smem, segs);
+        process_work<THREADS, UB, NT, FEAT>(a, t, smem, segs, tsegs);
     }
 }
 
@@ -536,21 +627,28 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
 // captured hipGraph.
 __global__ void k_reset_queue(uint32_t *q) { *q = 0; }
 
-template <bool P, int TH, int UB, bool NT>
+template <bool P, int TH, int UB, bool NT, int F>
 static hipError_t launch_variant(const gather_args &a, int grid_limit, hipStream_t stream) {
     if (!P) {
-        hipLaunchKernelGGL((k_gather<false, TH, UB, NT>), dim3(a.total_tiles), dim3(TH), 0, stream, a);
+        hipLaunchKernelGGL((k_gather<false, TH, UB, NT, F>), dim3(a.total_tiles), dim3(TH), 0, stream, a);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
     const uint32_t grid = std::min<uint32_t>(a.total_tiles, (uint32_t)std::max(1, grid_limit));
-    hipLaunchKernelGGL((k_gather<true, TH, UB, NT>), dim3(grid), dim3(TH), 0, stream, a);
+    hipLaunchKernelGGL((k_gather<true, TH, UB, NT, F>), dim3(grid), dim3(TH), 0, stream, a);
     return hipGetLastError();
+}
+
+template <bool P, int TH, int UB, bool NT>
+static hipError_t launch_feat(const gather_args &a, const launch_cfg &c, int gl, hipStream_t s) {
+    if (c.features == FEAT_COPY) return launch_variant<P, TH, UB, NT, FEAT_COPY>(a, gl, s);
+    if (c.features == FEAT_TRANS) return launch_variant<P, TH, UB, NT, FEAT_TRANS>(a, gl, s);
+    return launch_variant<P, TH, UB, NT, FEAT_ALL>(a, gl, s);
 }
 
 template <bool P, int TH, int UB>
 static hipError_t launch_nt(const gather_args &a, const launch_cfg &c, int gl, hipStream_t s) {
-    return c.nontemporal ? launch_variant<P, TH, UB, true>(a, gl, s) : launch_variant<P, TH, UB, false>(a, gl, s);
+    return c.nontemporal ? launch_feat<P, TH, UB, true>(a, c, gl, s) : launch_feat<P, TH, UB, false>(a, c, gl, s);
 }
 
 template <bool P, int TH>

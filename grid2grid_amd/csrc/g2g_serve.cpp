@@ -10,6 +10,7 @@
 #include "grid2grid_amd.h"
 #include "g2g_internal.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -144,6 +145,8 @@ int g2g_server_s::launch(slot &sl, std::vector<pending_req> &batch) {
         a.queue = sl.d_queue;
         launch_cfg c;
         c.persistent = true;
+        c.features = FEAT_COPY;
+        for (const work_desc &d : descs) c.features = std::max(c.features, request_features(d.kr.r[0]));
         if (launch_gather(a, c, internal_grid_limit(), sl.stream) != hipSuccess)
             return internal_fail(G2G_ERR_HIP, "server: gather launch failed");
     }

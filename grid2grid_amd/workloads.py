@@ -59,9 +59,9 @@ def random_request(rng, dtypes=(_lib.FLOAT32, _lib.FLOAT64, _lib.COMPLEX64), lo=
     return init, fin, dtype, trans
 
 
-def cfg4(n=2048, seed=1004):
+def cfg4(n=2048, seed=1004, p_trans=0.3):
     rng = np.random.default_rng(seed)
-    return [random_request(rng) for _ in range(n)]
+    return [random_request(rng, p_trans=p_trans) for _ in range(n)]
 
 
 def large_request():

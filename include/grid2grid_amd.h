@@ -184,6 +184,13 @@ int g2g_batch_execute(g2g_batch batch, void *stream);
  * is a kernel node, not a memset). */
 int g2g_batch_execute_graph(g2g_batch batch, void *stream);
 
+/* Autotuner of a batch (declared with g2g_tuning below): times the batch's own
+ * work-item sizes and kernel variants (always persistent) and keeps the
+ * fastest. A batch cuts every plan's piece stream at its own work-item size
+ * (default 128 KiB), independently of the plans' standalone configuration. */
+struct g2g_tuning;
+int g2g_batch_autotune(g2g_batch batch, void *stream, int reps, struct g2g_tuning *best);
+
 /* Bytes moved by one execution of the batch. */
 int g2g_batch_bytes(g2g_batch batch, int64_t *bytes);
 

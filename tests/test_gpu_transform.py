@@ -49,6 +49,10 @@ def _transposed_pairs():
         "arbT": (arbitrary_layout(rng, 130, 210, 4, 5, 4, lo=10, hi=80, pad=(0, 6), gap=(0, 17)),
                  arbitrary_layout(rng, 210, 130, 6, 3, 3, lo=10, hi=80, pad=(0, 6), gap=(0, 17))),
         "bcT_big_tiles": (scalapack_layout(160, 96, 64, 48, 2, 1, "C"), scalapack_layout(96, 160, 40, 64, 1, 2, "C")),
+        # pieces of 100 and 60 final rows by 128 and 44 final columns: both the
+        # LDS tile path and the element path of transposed pieces
+        "bcT_mixed_paths": (scalapack_layout(300, 260, 128, 100, 2, 2, "C"),
+                            scalapack_layout(260, 300, 100, 128, 2, 2, "C")),
     }
 
 

@@ -527,6 +527,7 @@ work_desc make_desc(const stage_table &t, const kreqs &kr, uint32_t tile_base) {
     d.total_bytes = t.total;
     d.n_tiles = t.n_tiles;
     d.tile_base = tile_base;
+    d.n_pieces = (uint32_t)t.n;
     d.kr = kr;
     return d;
 }
@@ -594,7 +595,8 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     G2G_TRY(union_spans(wf.cs, wi.cs, "columns", p->ucols));
     const int64_t n_ur = (int64_t)p->urows.size(), n_uc = (int64_t)p->ucols.size();
     const int64_t np = n_ur * n_uc;
-    if (np > 0xffffffffll) return fail(G2G_ERR_UNSUPPORTED, "too many pieces (%lld)", (long long)np);
+    // piece indices, plus the 64 examined at once by a wave, stay within 32 bits
+    if (np > 0xffffff00ll) return fail(G2G_ERR_UNSUPPORTED, "too many pieces (%lld)", (long long)np);
     for (const span &r : p->urows)
         if ((uint64_t)(r.end - r.start) * es > 0xffffffffull)
             return fail(G2G_ERR_UNSUPPORTED, "piece column exceeds 4 GiB");

@@ -132,7 +132,9 @@ struct work_desc {
     uint64_t total_bytes;
     uint32_t n_tiles;
     uint32_t tile_base;           // first global work index of this desc in a launch
-    kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1]
+    uint32_t n_pieces;
+    uint32_t pad;
+    kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1] (same element type)
 };
 
 // A launch: one desc passed by value, or a device table of descs with the

@@ -199,6 +199,16 @@ struct seg_entry {
     uint32_t pad;
 };
 constexpr int kMaxSeg = 64;
+// Transposed pieces with fewer final rows, or narrower final column ranges
+// (bytes), than these go to the element queue instead of LDS tiles.
+#ifndef G2G_TRANS_ELEM_ROWS
+#define G2G_TRANS_ELEM_ROWS 32
+#endif
+#ifndef G2G_TRANS_ELEM_COLB
+#define G2G_TRANS_ELEM_COLB 512
+#endif
+constexpr uint32_t kTransElemRows = G2G_TRANS_ELEM_ROWS;
+constexpr uint32_t kTransElemColBytes = G2G_TRANS_ELEM_COLB;
 
 __device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t mul) {
     const uint32_t l = ceil_log2(d);
@@ -362,31 +372,37 @@ __device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::ty
     }
 }
 
-constexpr uint32_t kTransRows = 64;  // final rows (storage columns) per transpose tile
-constexpr uint32_t kSmemBytes = kTransRows * (256 + 16);
+constexpr uint32_t kTileBytes = 16384;  // data bytes of one transpose tile
+constexpr uint32_t kSmemBytes = kTileBytes + 64 * 16;  // plus one padding element per tile row
 
 // Transposing copy: destination(i, j) = op(source(j, i)) for final rows
 // i < rows and final columns j < ncols. Source element (i, j) lies at
-// s + (i*lds + j)*ES: storage rows run along final columns. A tile is
-// TR = 256/ES final columns (a 256-byte stretch of each storage column) by 64
-// final rows. Both phases move 16-byte vectors (dword-aligned addresses), so
-// every 16 lanes of a wave read or write one whole 256-byte segment; LDS holds
-// the tile in storage order, rows padded by one element against bank
-// conflicts. Vectors that would run past the piece move element by element.
-template <int THREADS, bool NT, int ES, bool SCALE>
-__device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
+// s + (i*lds + j)*ES: storage rows run along final columns. A tile is TC final
+// rows (TC = 64, or 32 / 16 for pieces with fewer rows) by TR = 16384/(TC*ES)
+// final columns. Phase 1 reads TR*ES contiguous bytes (256-1024) of each of
+// the TC source storage columns; phase 2 writes TC*ES contiguous bytes of
+// each of the TR destination columns. Both phases move 16-byte vectors
+// (dword-aligned addresses); LDS holds the tile in storage order, rows padded
+// by one element against bank conflicts. Vectors that would run past the
+// piece move element by element.
+// Not inlined: a called function keeps the registers of this path out of the
+// queue loops of the gather (fewer registers per wave, more waves per CU).
+template <int THREADS, bool NT, int ES, uint32_t TC, bool SCALE>
+__device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restrict__ s,
                                                 gbyte *__restrict__ d, uint64_t lds_b,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
                                                 const request &rq, unsigned char *smem) {
     using T = typename bits_t<ES>::type;
-    constexpr uint32_t VE = 16 / ES;             // elements per vector
-    constexpr uint32_t TR = 256 / ES;            // final columns per tile
-    constexpr uint32_t TC = kTransRows;          // final rows per tile
-    constexpr uint32_t LDW = TR + 1;             // LDS row length in elements
-    constexpr uint32_t W = TC * ES / 16;         // vectors per final column of a tile
-    constexpr uint32_t WL = ES == 4 ? 4 : ES == 8 ? 5 : 6;  // log2(W)
-    constexpr uint32_t PER = TC * 16 / THREADS;  // vectors per thread per phase
-    T *tile = reinterpret_cast<T *>(smem);       // tile[c * LDW + r]
+    constexpr uint32_t VE = 16 / ES;                     // elements per vector
+    constexpr uint32_t TR = kTileBytes / (TC * ES);      // final columns per tile
+    constexpr uint32_t LDW = TR + 1;                     // LDS row length in elements
+    constexpr uint32_t RV = TR / VE;                     // vectors per source run
+    constexpr uint32_t RVL = __builtin_ctz(RV);
+    constexpr uint32_t W = TC * ES / 16;                 // vectors per final column of a tile
+    constexpr uint32_t WL = __builtin_ctz(W);
+    constexpr uint32_t PER = kTileBytes / 16 / THREADS;  // vectors per thread per phase
+    static_assert(TC * LDW * ES <= kSmemBytes, "transpose tile exceeds its LDS");
+    T *tile = reinterpret_cast<T *>(smem);               // tile[c * LDW + r]
     const bool conj = (rq.flags & REQ_CONJ) != 0;
     const bool scale = SCALE && (rq.flags & REQ_SCALE) != 0;
     for (uint32_t jc = 0; jc < ncols; jc += TR) {
@@ -394,7 +410,7 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
 #pragma unroll
             for (uint32_t k = 0; k < PER; ++k) {
                 const uint32_t q = threadIdx.x + k * THREADS;
-                const uint32_t c = q >> 4, r0 = (q & 15) * VE;
+                const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
                 const uint32_t i = ic + c, j0 = jc + r0;
                 if (i < rows && j0 < ncols) {
                     const gbyte *p = s + i * lds_b + (uint64_t)j0 * ES;
@@ -444,6 +460,20 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
     }
 }
 
+// Tile height for a transposed piece of `rows` final rows.
+template <int THREADS, bool NT, int ES, bool SCALE>
+__device__ __forceinline__ void copy_transposed_rows(const gbyte *__restrict__ s,
+                                                     gbyte *__restrict__ d, uint64_t lds_b,
+                                                     uint64_t ldd_b, uint32_t rows, uint32_t ncols,
+                                                     const request &rq, unsigned char *smem) {
+    if (rows > 32)
+        copy_transposed<THREADS, NT, ES, 64, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+    else if (rows > 16)
+        copy_transposed<THREADS, NT, ES, 32, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+    else
+        copy_transposed<THREADS, NT, ES, 16, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+}
+
 // Transposed or scaled pieces: moved by the whole workgroup right away.
 template <int THREADS, bool NT, int FEAT>
 __device__ __forceinline__ void process_piece(const piece &pc, const request &rq, uint32_t c0,
@@ -456,11 +486,11 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     if (rq.flags & REQ_TRANS) {
         const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * es;
         if (es == 4)
-            copy_transposed<THREADS, NT, 4, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed_rows<THREADS, NT, 4, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else if (es == 8)
-            copy_transposed<THREADS, NT, 8, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed_rows<THREADS, NT, 8, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else
-            copy_transposed<THREADS, NT, 16, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            copy_transposed_rows<THREADS, NT, 16, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         return;
     }
     if constexpr (FEAT != FEAT_ALL) return;  // the host never sends scaling here
@@ -473,118 +503,145 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
         copy_cols_scaled<THREADS, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq);
 }
 
+// Columns [c0, c1) of a piece that belong to the work item covering stream
+// bytes [lo, hi), for a piece occupying stream bytes [ps, pe): the columns
+// (groups of 256/es columns for transposed pieces) whose first byte lies in
+// [lo, hi).
+__device__ __forceinline__ void piece_cols(uint32_t rows, uint32_t cols, uint32_t flags, uint32_t es,
+                                           uint64_t ps, uint64_t pe, uint64_t lo, uint64_t hi,
+                                           uint32_t &c0, uint32_t &c1) {
+    c0 = 0;
+    c1 = cols;
+    if (ps < lo || pe > hi) {  // piece shared with neighbouring work items
+        const uint64_t q = (flags & REQ_TRANS) ? 256u / es : 1u;
+        const uint64_t qb = (uint64_t)rows * es * q;
+        const uint64_t a0 = lo > ps ? ((lo - ps + qb - 1) / qb) * q : 0;
+        const uint64_t a1 = ((hi - ps + qb - 1) / qb) * q;
+        c0 = (uint32_t)min(a0, (uint64_t)cols);
+        c1 = (uint32_t)min(a1, (uint64_t)cols);
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+enum : uint32_t { PK_NONE = 0, PK_VEC = 1, PK_ELEM = 2, PK_DIRECT = 3 };
+
 // Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
 // stream (pieces concatenated in table order, each column-major). It moves
 // every column (every group of 256/sizeof(T) columns for transposed pieces)
 // whose first byte lies in that range, so work items partition the columns
-// exactly. Untransposed, unscaled column runs are queued (up to kMaxSeg
-// pieces) and moved together by flat_copy; small transposed pieces are queued
-// likewise for flat_copy_elems; large transposed and scaled pieces are moved
-// as they come.
+// exactly. Pieces are examined 64 at a time, one per lane (every wave does the
+// same arithmetic; wave 0 writes the queues): untransposed, unscaled column
+// runs are queued for flat_copy and small transposed pieces for
+// flat_copy_elems; once the queues are drained, the pieces that are moved on
+// their own (large transposed pieces, scaled pieces) follow one by one, each
+// by the whole workgroup. Pieces write disjoint destinations, so their order
+// does not matter.
 template <int THREADS, int UB, bool NT, int FEAT>
 __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, unsigned char *smem,
                                              seg_entry *segs, seg_entry *tsegs) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
     const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t es = a.kr.r[0].esize;  // both requests of a desc share the element type
+    const uint32_t n = a.n_pieces;
     uint32_t p = a.first_piece[t];
-    uint64_t pstart = p ? a.pend[p - 1] : 0;
-    bool more = true;
-    while (more) {
-        uint32_t nseg = 0, total = 0, ntseg = 0, ttotal = 0, tes = 4;
-        for (;;) {
-            if (pstart >= hi) {
-                more = false;
-                break;
-            }
-            if (nseg == kMaxSeg || ntseg == kMaxSeg) break;
-            const piece pc = a.pieces[p];
-            const uint64_t pend = a.pend[p];
-            const request rq = pc.req ? a.kr.r[1] : a.kr.r[0];
-            const uint32_t es = rq.esize;
-            const uint32_t colb = pc.rows * es;
-            uint32_t c0 = 0, c1 = pc.cols;
-            if (pstart < lo || pend > hi) {  // piece shared with neighbouring work items
-                const uint64_t q = (rq.flags & REQ_TRANS) ? 256u / es : 1u;
-                const uint64_t qb = (uint64_t)colb * q;
-                const uint64_t a0 = lo > pstart ? ((lo - pstart + qb - 1) / qb) * q : 0;
-                const uint64_t a1 = ((hi - pstart + qb - 1) / qb) * q;
-                c0 = (uint32_t)min(a0, (uint64_t)pc.cols);
-                c1 = (uint32_t)min(a1, (uint64_t)pc.cols);
-            }
-            ++p;
-            pstart = pend;
-            if (c0 >= c1) continue;
-            const bool trans = (rq.flags & REQ_TRANS) != 0;
-            const uint64_t lds_b = (uint64_t)pc.lds * es, ldd_b = (uint64_t)pc.ldd * es;
-            if constexpr (FEAT == FEAT_COPY) {
-                // the copy-only variant is launched for untransposed, unscaled
-                // requests only; anything else is left alone rather than misread
-                if (rq.flags & (REQ_TRANS | REQ_SCALE)) continue;
-            } else {
-                if (rq.flags & REQ_SCALE) {
-                    process_piece<THREADS, NT, FEAT>(pc, rq, c0, c1, smem);
-                    continue;
-                }
-                if (trans) {
+    for (;;) {
+        if (p >= n || (p ? a.pend[p - 1] : 0) >= hi) break;
+        const uint32_t q = p + lane;
+        uint32_t kind = PK_NONE, c0 = 0, c1 = 0, rows = 0, flags = 0, vpc = 0;
+        piece pc;
+        bool in = q < n;
+        if (in) {
+            pc = a.pieces[q];
+            const uint64_t ps = q ? a.pend[q - 1] : 0;
+            const uint64_t pe = a.pend[q];
+            in = ps < hi;
+            flags = pc.req ? a.kr.r[1].flags : a.kr.r[0].flags;
+            rows = pc.rows;
+            piece_cols(rows, pc.cols, flags, es, ps, pe, lo, hi, c0, c1);
+            if (in && c0 < c1) {
+                const bool trans = (flags & REQ_TRANS) != 0;
+                if (FEAT == FEAT_COPY) {
+                    // the copy-only variant is launched for untransposed,
+                    // unscaled requests only; anything else is left alone
+                    kind = (flags & (REQ_TRANS | REQ_SCALE)) ? PK_NONE : PK_VEC;
+                } else if (flags & REQ_SCALE) {
+                    kind = PK_DIRECT;
+                } else if (trans) {
                     // transposed pieces that would leave an LDS tile mostly
-                    // empty move element by element in their own queue; a
-                    // queue holds one element size
-                    const bool small_t = pc.rows < kTransRows || (c1 - c0) < 256u / es;
-                    if (!small_t || (ntseg && es != tes)) {
-                        process_piece<THREADS, NT, FEAT>(pc, rq, c0, c1, smem);
-                        continue;
-                    }
-                    if (threadIdx.x == 0) {
-                        seg_entry &e = tsegs[ntseg];
-                        e.s = pc.src + (uint64_t)c0 * es;
-                        e.d = pc.dst + (uint64_t)c0 * ldd_b;
-                        e.lds_b = lds_b;
-                        e.ldd_b = ldd_b;
-                        e.colb = colb;
-                        e.vpc = pc.rows;
-                        e.slot0 = ttotal;
-                        e.vmul = div_magic(pc.rows);
-                        e.kind = (rq.flags & REQ_CONJ) ? SEG_CONJ : SEG_TRANS;
-                        e.pad = 0;
-                    }
-                    tes = es;
-                    ttotal += (c1 - c0) * pc.rows;
-                    ++ntseg;
-                    continue;
+                    // empty move element by element
+                    kind = (rows < kTransElemRows || (c1 - c0) * es < kTransElemColBytes) ? PK_ELEM : PK_DIRECT;
+                } else {
+                    kind = PK_VEC;
                 }
+                vpc = kind == PK_VEC ? (rows * es + 15) >> 4 : rows;
             }
-            if (threadIdx.x == 0) {
-                seg_entry &e = segs[nseg];
-                e.s = pc.src + (uint64_t)c0 * lds_b;
-                e.d = pc.dst + (uint64_t)c0 * ldd_b;
-                e.lds_b = lds_b;
-                e.ldd_b = ldd_b;
-                e.colb = colb;
-                e.vpc = (colb + 15) >> 4;
-                e.slot0 = total;
-                e.vmul = pc.vmul;
-                e.kind = SEG_VEC;
-                e.pad = 0;
-            }
-            total += (c1 - c0) * ((colb + 15) >> 4);
-            ++nseg;
         }
+        // pieces past the work item come after every piece inside it
+        const uint64_t out_mask = __ballot(!in);
+        const uint32_t end = out_mask ? (uint32_t)__ffsll((unsigned long long)out_mask) - 1 : 64u;
+        const bool is_v = kind == PK_VEC, is_e = kind == PK_ELEM;
+        const uint64_t mv = __ballot(is_v), me = __ballot(is_e);
+        uint64_t md = __ballot(kind == PK_DIRECT);
+        const uint32_t nseg = (uint32_t)__popcll(mv), ntseg = (uint32_t)__popcll(me);
         if (nseg | ntseg) {
-            if (threadIdx.x == 0) {  // sentinels
-                segs[nseg].slot0 = total;
-                tsegs[ntseg].slot0 = ttotal;
+            const uint32_t slots = (is_v || is_e) ? (c1 - c0) * vpc : 0u;
+            const uint32_t sv = wave_scan_u32(is_v ? slots : 0u, lane);
+            const uint32_t se = wave_scan_u32(is_e ? slots : 0u, lane);
+            const uint32_t total = __shfl(sv, 63, 64), ttotal = __shfl(se, 63, 64);
+            if (threadIdx.x < 64) {
+                const uint64_t below = (1ull << lane) - 1;
+                if (is_v || is_e) {
+                    const uint64_t ldd_b = (uint64_t)pc.ldd * es, lds_b = (uint64_t)pc.lds * es;
+                    seg_entry &e = is_v ? segs[__popcll(mv & below)] : tsegs[__popcll(me & below)];
+                    e.s = pc.src + (uint64_t)c0 * (is_v ? lds_b : es);
+                    e.d = pc.dst + (uint64_t)c0 * ldd_b;
+                    e.lds_b = lds_b;
+                    e.ldd_b = ldd_b;
+                    e.colb = rows * es;
+                    e.vpc = vpc;
+                    e.slot0 = is_v ? sv - slots : se - slots;
+                    e.vmul = is_v ? pc.vmul : div_magic(vpc);
+                    e.kind = is_v ? SEG_VEC : (flags & REQ_CONJ) ? SEG_CONJ : SEG_TRANS;
+                    e.pad = 0;
+                }
+                if (lane == 0) {  // sentinels
+                    segs[nseg].slot0 = total;
+                    tsegs[ntseg].slot0 = ttotal;
+                }
             }
             __syncthreads();
             if (nseg) flat_copy<THREADS, UB, NT>(segs, total);
             if constexpr (FEAT != FEAT_COPY) {
                 if (ntseg) {
-                    if (tes == 4) flat_copy_elems<THREADS, NT, 4>(tsegs, ttotal);
-                    else if (tes == 8) flat_copy_elems<THREADS, NT, 8>(tsegs, ttotal);
+                    if (es == 4) flat_copy_elems<THREADS, NT, 4>(tsegs, ttotal);
+                    else if (es == 8) flat_copy_elems<THREADS, NT, 8>(tsegs, ttotal);
                     else flat_copy_elems<THREADS, NT, 16>(tsegs, ttotal);
                 }
             }
-            __syncthreads();
+            __syncthreads();  // the queues are rewritten by the next examination
         }
+        if constexpr (FEAT != FEAT_COPY) {
+            while (md) {
+                const uint32_t d = p + (uint32_t)__ffsll((unsigned long long)md) - 1;
+                md &= md - 1;
+                const piece dp = a.pieces[d];
+                const request &rq = dp.req ? a.kr.r[1] : a.kr.r[0];
+                uint32_t d0, d1;
+                piece_cols(dp.rows, dp.cols, rq.flags, es, d ? a.pend[d - 1] : 0, a.pend[d], lo, hi, d0, d1);
+                process_piece<THREADS, NT, FEAT>(dp, rq, d0, d1, smem);
+            }
+        }
+        if (end < 64) break;
+        p += 64;
     }
 }
 

@@ -53,6 +53,13 @@ def _transposed_pairs():
         # LDS tile path and the element path of transposed pieces
         "bcT_mixed_paths": (scalapack_layout(300, 260, 128, 100, 2, 2, "C"),
                             scalapack_layout(260, 300, 100, 128, 2, 2, "C")),
+        # pieces of 8-24 final rows by up to 96 final columns: 16-row tiles
+        # and the element queue
+        "bcT_short_tiles": (scalapack_layout(400, 300, 96, 40, 2, 2, "C"),
+                            scalapack_layout(300, 400, 24, 160, 2, 2, "C")),
+        # pieces of 16-40 final rows by up to 200 final columns: 32- and 64-row tiles
+        "bcT_mid_tiles": (scalapack_layout(400, 300, 200, 56, 2, 2, "C"),
+                          scalapack_layout(300, 400, 40, 256, 2, 2, "C")),
     }
 
 

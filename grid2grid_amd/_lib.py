@@ -59,6 +59,10 @@ class G2GServerStats(ctypes.Structure):
                 ("peak_in_flight_bytes", ctypes.c_int64)]
 
 
+class G2GEdge(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_int32), ("dest", ctypes.c_int32), ("weight", ctypes.c_int64)]
+
+
 class G2GError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
@@ -103,6 +107,9 @@ PROTOTYPES = {
     "g2g_server_drain": (_i, [_vp]),
     "g2g_server_get_stats": (_i, [_vp, ctypes.POINTER(G2GServerStats)]),
     "g2g_server_destroy": (_i, [_vp]),
+    "g2g_comm_volume": (_i, [ctypes.POINTER(G2GLayout), ctypes.POINTER(G2GLayout), ctypes.POINTER(G2GEdge),
+                             _i64, _p64, _vp]),
+    "g2g_optimal_reordering": (_i, [ctypes.POINTER(G2GEdge), _i64, ctypes.c_int32, _p32, _p32]),
     "g2g_fill_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, _vp]),
     "g2g_verify_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, ctypes.c_char, _i64, _vp,
                                _p64]),

@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import check, G2GPlanInfo, DTYPE_SIZE
+from ._lib import check, G2GPlanInfo, G2GEdge, DTYPE_SIZE
 
 
 def _scalar_bytes(dtype, value):
@@ -242,3 +242,30 @@ def verify_layout(layout, base, dtype, seed, trans, a_n_cols, stream=None):
     check(_lib.load().g2g_verify_layout(d.ref(), dtype, ctypes.c_uint64(seed), trans.encode(), a_n_cols,
                                         _stream(stream), ctypes.byref(bad)))
     return bad.value
+
+
+def comm_volume(initial, final, stream=None):
+    """Elements exchanged between each unordered pair of ranks {a, b}, a <= b
+    (a == b: elements a rank keeps), as {(a, b): count}, computed on the
+    device (reference: transform.cpp:9-39, utils.hpp:91-139)."""
+    lib = _lib.load()
+    di, df = initial.descriptor(0, 1), final.descriptor(0, 1)
+    n = ctypes.c_int64()
+    check(lib.g2g_comm_volume(di.ref(), df.ref(), None, 0, ctypes.byref(n), _stream(stream)))
+    edges = (G2GEdge * max(1, n.value))()
+    check(lib.g2g_comm_volume(di.ref(), df.ref(), edges, n.value, ctypes.byref(n), _stream(stream)))
+    return {(int(e.src), int(e.dest)): int(e.weight) for e in edges[:n.value]}
+
+
+def optimal_reordering(volume, n_ranks):
+    """Greedy rank relabelling on a volume {(a, b): count}
+    (reference: ranks_reordering.cpp:4-61). Returns (permutation, reordered)."""
+    lib = _lib.load()
+    items = list(volume.items())
+    edges = (G2GEdge * max(1, len(items)))()
+    for k, ((a, b), w) in enumerate(items):
+        edges[k].src, edges[k].dest, edges[k].weight = int(a), int(b), int(w)
+    perm = (ctypes.c_int32 * n_ranks)()
+    reordered = ctypes.c_int32()
+    check(lib.g2g_optimal_reordering(edges, len(items), n_ranks, perm, ctypes.byref(reordered)))
+    return [int(x) for x in perm], bool(reordered.value)

@@ -952,6 +952,26 @@ int internal_fail(int code, const char *fmt, ...) {
     return code;
 }
 
+int internal_check_grid(const char *what, const g2g_layout *l) {
+    if (!l) return fail(G2G_ERR_INVALID, "%s layout is NULL", what);
+    G2G_TRY(check_splits(what, l->rows_split, l->n_rows_split));
+    G2G_TRY(check_splits(what, l->cols_split, l->n_cols_split));
+    if (l->n_ranks < 1 || l->n_ranks > 65535)
+        return fail(G2G_ERR_INVALID, "%s: n_ranks must be in [1, 65535]", what);
+    if (!l->owners) return fail(G2G_ERR_INVALID, "%s: owners is NULL", what);
+    const int64_t cells = (int64_t)(l->n_rows_split - 1) * (l->n_cols_split - 1);
+    for (int64_t k = 0; k < cells; ++k)
+        if (l->owners[k] < 0 || l->owners[k] >= l->n_ranks)
+            return fail(G2G_ERR_INVALID, "%s: owner %d of cell %lld outside [0, %d)", what, l->owners[k],
+                        (long long)k, l->n_ranks);
+    return G2G_OK;
+}
+
+int internal_union_spans(const std::vector<int32_t> &f, const std::vector<int32_t> &i, const char *dim,
+                         std::vector<span> &out) {
+    return union_spans(f, i, dim, out);
+}
+
 }  // namespace g2g
 
 

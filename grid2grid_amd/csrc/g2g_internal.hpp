@@ -187,6 +187,12 @@ int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas,
 int internal_grid_limit();
 // Records the thread's last error message and returns code.
 int internal_fail(int code, const char *fmt, ...);
+// Validates the splits, owners and n_ranks of a layout (its blocks are not read).
+int internal_check_grid(const char *what, const struct g2g_layout *l);
+// The union of two split vectors covering the same range, as intervals
+// tagged with the block of each vector they lie in (f: `fblk`, i: `iblk`).
+int internal_union_spans(const std::vector<int32_t> &f, const std::vector<int32_t> &i, const char *dim,
+                         std::vector<span> &out);
 
 // Test-support kernels.
 struct fill_block {

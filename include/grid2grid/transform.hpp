@@ -12,7 +12,9 @@
 
 #include <grid2grid_amd.h>
 
+#include <grid2grid/comm_volume.hpp>
 #include <grid2grid/grid_layout.hpp>
+#include <grid2grid/ranks_reordering.hpp>
 
 #include <complex>
 #include <cstdint>
@@ -173,6 +175,44 @@ template <typename T>
 void transform(std::vector<layout_ref<T>> &initial_layouts, std::vector<layout_ref<T>> &final_layouts,
                const virtual_comm &comm = {}) {
     transform<T>(initial_layouts, final_layouts, nullptr, nullptr, nullptr, comm);
+}
+
+// Elements exchanged between each pair of ranks when data moves from g_init
+// to g_final (transform.hpp:47, transform.cpp:9-39), computed on the device
+// (g2g_comm_volume). Owners are taken after any rank reordering, as
+// assigned_grid2D::owner gives them.
+inline comm_volume communication_volume(assigned_grid2D &g_init, assigned_grid2D &g_final,
+                                        void *stream = nullptr) {
+    struct grid_desc {
+        std::vector<int32_t> rs, cs, owners;
+        g2g_layout l{};
+        explicit grid_desc(const assigned_grid2D &g) {
+            const grid2D &gr = g.grid();
+            rs.assign(gr.rows_split.begin(), gr.rows_split.end());
+            cs.assign(gr.cols_split.begin(), gr.cols_split.end());
+            owners.resize(size_t(gr.n_rows) * gr.n_cols);
+            for (int i = 0; i < gr.n_rows; ++i)
+                for (int j = 0; j < gr.n_cols; ++j) owners[size_t(i) * gr.n_cols + j] = g.owner(i, j);
+            l.n_rows_split = int32_t(rs.size());
+            l.n_cols_split = int32_t(cs.size());
+            l.rows_split = rs.data();
+            l.cols_split = cs.data();
+            l.owners = owners.data();
+            l.n_ranks = g.num_ranks();
+        }
+    };
+    grid_desc di(g_init), df(g_final);
+    int64_t n = 0;
+    detail::check(g2g_comm_volume(&di.l, &df.l, nullptr, 0, &n, stream));
+    std::vector<g2g_edge> edges((size_t)n);
+    detail::check(g2g_comm_volume(&di.l, &df.l, edges.data(), n, &n, stream));
+    comm_volume::volume_t v;
+    for (const g2g_edge &e : edges) {
+        if (e.weight > 0x7fffffffll)
+            throw std::runtime_error("grid2grid: communication volume of a rank pair exceeds int");
+        v[edge_t{e.src, e.dest}] = int(e.weight);
+    }
+    return comm_volume(std::move(v));
 }
 
 }  // namespace grid2grid

@@ -273,6 +273,42 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
  * layout is written). *best receives the chosen configuration. */
 int g2g_plan_autotune(g2g_plan plan, void *stream, int reps, g2g_tuning *best);
 
+/* ------------------------------------ communication volume, rank relabelling */
+
+/* Elements exchanged between one unordered pair of ranks {src, dest},
+ * src <= dest (src == dest: elements that stay on the rank). */
+typedef struct g2g_edge {
+    int32_t src;
+    int32_t dest;
+    int64_t weight;
+} g2g_edge;
+
+/* Replaces grid2grid::communication_volume(assigned_grid2D&, assigned_grid2D&)
+ * (transform.hpp:47, transform.cpp:9-39, utils.hpp:91-139): for every cell of
+ * the initial grid and every final cell it overlaps, the overlap's element
+ * count is added to the pair {initial owner, final owner}. Only the splits,
+ * owners and n_ranks of the two layouts are read (blocks may be absent).
+ * Computed on the device (one key per cell of the split union, radix sort,
+ * reduction by key); synchronous on `stream`. *n_edges receives the number of
+ * pairs with a nonzero volume; the first min(capacity, *n_edges) of them are
+ * written to `edges` (may be NULL) in ascending (src, dest) order. */
+int g2g_comm_volume(const g2g_layout *initial, const g2g_layout *final, g2g_edge *edges,
+                    int64_t capacity, int64_t *n_edges, void *stream);
+
+/* Replaces grid2grid::optimal_reordering(comm_volume&, int, bool&)
+ * (ranks_reordering.hpp:7, ranks_reordering.cpp:4-61): greedy matching of
+ * ranks on the volume. Edge {a, b}, a != b, weighs vol(a,b) - vol(a,a) -
+ * vol(b,b); edge {a, a} weighs 1; edges weighing 0 are dropped; edges are taken
+ * by decreasing weight when both ends are still free (permutation[a] = b,
+ * permutation[b] = a), and both ends are then taken. Equal weights go in
+ * ascending (a, b) order, and every rank that occurs in an edge has its {a, a}
+ * edge (the reference leaves both points to its hash map). Edges may come in
+ * any order, with src > dest and repeats (summed, as comm_volume::operator+=
+ * does). Host-only. `permutation` has n_ranks entries; *reordered = 1 when some
+ * rank is mapped to another. */
+int g2g_optimal_reordering(const g2g_edge *edges, int64_t n_edges, int32_t n_ranks,
+                           int32_t *permutation, int32_t *reordered);
+
 /* ------------------------------------------------ test-support kernels */
 
 /* Fill every block of `layout` with the seeded value v(seed, i, j) of its

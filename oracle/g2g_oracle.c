@@ -779,3 +779,118 @@ int64_t orc_check_layout(const orc_layout *fin, int dtype, uint64_t seed, char t
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------ */
+/* communication volume and rank relabelling                            */
+
+/* transform.cpp:9-39 and utils.hpp:91-139. */
+int orc_comm_volume(const orc_layout *init, const orc_layout *fin, int64_t *vol) {
+    const int n = orc_n_ranks(init, fin);
+    const int nbr = init->n_rows_split - 1, nbc = init->n_cols_split - 1;
+    const int fnc = fin->n_cols_split - 1;
+    int rc = -1;
+    int32_t *rcov = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)nbr);
+    int32_t *ccov = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)nbc);
+    if (!rcov || !ccov) goto done;
+    memset(vol, 0, sizeof(int64_t) * (size_t)n * n);
+    /* grid_cover g_cover(g_init.grid(), g_final.grid()) */
+    if (orc_decomp_cover(init->rows_split, init->n_rows_split, fin->rows_split, fin->n_rows_split, rcov))
+        goto done;
+    if (orc_decomp_cover(init->cols_split, init->n_cols_split, fin->cols_split, fin->n_cols_split, ccov))
+        goto done;
+    for (int i = 0; i < nbr; ++i) {
+        for (int j = 0; j < nbc; ++j) {
+            const int rank = init->owners[(size_t)i * nbc + j];
+            /* rank_to_comm_vol_for_block(g_init, {i, j}, g_cover, g_final) */
+            const int row_first = rcov[2 * i], row_last = rcov[2 * i + 1];
+            const int col_first = ccov[2 * j], col_last = ccov[2 * j + 1];
+            const int r_end = init->rows_split[i + 1], c_end = init->cols_split[j + 1];
+            int row_start = init->rows_split[i];
+            for (int fi = row_first; fi < row_last; ++fi) {
+                const int row_end = fin->rows_split[fi + 1] < r_end ? fin->rows_split[fi + 1] : r_end;
+                int col_start = init->cols_split[j];
+                for (int fj = col_first; fj < col_last; ++fj) {
+                    const int target = fin->owners[(size_t)fi * fnc + fj];
+                    const int col_end = fin->cols_split[fj + 1] < c_end ? fin->cols_split[fj + 1] : c_end;
+                    const int64_t size = (int64_t)(row_end - row_start) * (col_end - col_start);
+                    if (size) {
+                        /* weights[{min, max}] += weight */
+                        const int a = rank < target ? rank : target;
+                        const int b = rank < target ? target : rank;
+                        vol[(size_t)a * n + b] += size;
+                    }
+                    col_start = col_end;
+                }
+                row_start = row_end;
+            }
+        }
+    }
+    rc = 0;
+done:
+    free(rcov);
+    free(ccov);
+    return rc;
+}
+
+typedef struct orc_wedge {
+    int a, b;
+    int64_t w;
+} orc_wedge;
+
+/* decreasing weight, then ascending (a, b) */
+static int wedge_cmp(const void *x, const void *y) {
+    const orc_wedge *p = (const orc_wedge *)x, *q = (const orc_wedge *)y;
+    if (p->w != q->w) return p->w > q->w ? -1 : 1;
+    if (p->a != q->a) return p->a < q->a ? -1 : 1;
+    if (p->b != q->b) return p->b < q->b ? -1 : 1;
+    return 0;
+}
+
+/* ranks_reordering.cpp:4-61. */
+int orc_optimal_reordering(const int64_t *vol, int n, int32_t *perm, int *reordered) {
+    if (n < 1) return -1;
+    int rc = -1;
+    char *visited = (char *)calloc((size_t)n, 1);
+    char *occurs = (char *)calloc((size_t)n, 1);
+    orc_wedge *e = (orc_wedge *)malloc(sizeof(orc_wedge) * ((size_t)n * (n + 1) / 2 + 1));
+    if (!visited || !occurs || !e) goto done;
+    for (int a = 0; a < n; ++a) perm[a] = a; /* identity permutation */
+    *reordered = 0;
+    for (int a = 0; a < n; ++a)
+        for (int b = a; b < n; ++b)
+            if (vol[(size_t)a * n + b] > 0) occurs[a] = occurs[b] = 1;
+    size_t ne = 0;
+    for (int a = 0; a < n; ++a) {
+        for (int b = a; b < n; ++b) {
+            const int64_t v = vol[(size_t)a * n + b];
+            int64_t w;
+            if (a == b) {
+                if (!occurs[a]) continue;
+                w = 2 * v + 1 - 2 * v; /* w *= 2; ++w; minus both self volumes */
+            } else {
+                if (v <= 0) continue;
+                w = v - vol[(size_t)a * n + a] - vol[(size_t)b * n + b];
+            }
+            if (w) {
+                e[ne].a = a;
+                e[ne].b = b;
+                e[ne].w = w;
+                ++ne;
+            }
+        }
+    }
+    qsort(e, ne, sizeof(orc_wedge), wedge_cmp);
+    for (size_t k = 0; k < ne; ++k) {
+        if (visited[e[k].a] || visited[e[k].b]) continue;
+        perm[e[k].a] = e[k].b;
+        perm[e[k].b] = e[k].a;
+        if (e[k].a != e[k].b) *reordered = 1;
+        visited[e[k].a] = visited[e[k].b] = 1;
+    }
+    rc = 0;
+done:
+    free(visited);
+    free(occurs);
+    free(e);
+    return rc;
+}

@@ -107,6 +107,37 @@ int orc_transform_batched(int n_pairs, const orc_layout *inits, const orc_layout
                           int dtype, const char *trans, const void *alphas,
                           const void *betas);
 
+/* transform.cpp:9-39 (communication_volume) with utils.hpp:91-139
+ * (rank_to_comm_vol_for_block): for every cell (i, j) of the initial grid,
+ * owned by rank r, and every final cell it overlaps (found with the cover of
+ * grid_cover.cpp:54-121), owned by rank s, the overlap's element count is
+ * added to the unordered pair {min(r,s), max(r,s)}. vol is a dense
+ * n x n row-major matrix, n = max(init->n_ranks, fin->n_ranks), written in
+ * its upper triangle (vol[a*n+b], a <= b; the rest is zeroed). Only the
+ * splits, owners and n_ranks of the layouts are read. Returns 0 or -1.
+ * Pinned: the reference's own cover and owners via oracle/_ref; the total
+ * equals m*n and vol[a*n+a] equals a brute-force count over elements in tests. */
+int orc_comm_volume(const orc_layout *init, const orc_layout *fin, int64_t *vol);
+
+/* ranks_reordering.cpp:4-61 (optimal_reordering): greedy matching on the
+ * volume vol (dense n x n, upper triangle as written by orc_comm_volume).
+ * Candidate edges are the pairs {a, b} with vol > 0. Adjusted weight:
+ * vol(a,b) - vol(a,a) - vol(b,b) for a != b, and 2*vol(a,a)+1 - 2*vol(a,a)
+ * = 1 for a == b; edges of adjusted weight 0 are dropped. Edges are visited by
+ * decreasing adjusted weight; an edge whose two ends are both unvisited sets
+ * perm[a] = b and perm[b] = a, and both ends become visited either way.
+ * Two points the reference leaves unspecified are fixed here: equal adjusted
+ * weights are visited in ascending (a, b) order (the reference sorts on the
+ * weight alone, over an unordered_map's iteration order), and a rank that
+ * keeps no elements still gets its {a, a} edge of weight 1 when it occurs in
+ * any edge (the reference inserts that entry while iterating the map, so
+ * whether it is visited depends on the map's buckets).
+ * *reordered = 1 when some a != b was matched. Returns 0 or -1.
+ * Pinned: the reference's optimal_reordering via oracle/_ref on volumes where
+ * the two unspecified points cannot change the result (every rank keeps
+ * elements, adjusted weights above 1 distinct), and brute-force properties. */
+int orc_optimal_reordering(const int64_t *vol, int n, int32_t *perm, int *reordered);
+
 /* scalapack_layout.cpp:152-177 (line_split). Returns number of splits written. */
 int orc_line_split(int begin, int end, int blk_len, int32_t *out, int max_out);
 

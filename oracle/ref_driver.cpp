@@ -22,11 +22,16 @@
 //     counts/dspls;
 //   * grid_layout::transpose_or_conjugate (grid_layout.hpp:18-25): the
 //     reference's assigned_grid2D::transpose and
-//     local_blocks::transpose_or_conjugate.
+//     local_blocks::transpose_or_conjugate;
+//   * communication_volume (transform.cpp:9-39): the per-block loop of
+//     utils.hpp:91-139 restated on the reference's cover and owners, summed
+//     into the reference's comm_volume;
+//   * optimal_reordering: the reference's ranks_reordering.cpp itself.
 #include <grid2grid/block.hpp>
 #include <grid2grid/communication_data.hpp>
 #include <grid2grid/grid2D.hpp>
 #include <grid2grid/grid_cover.hpp>
+#include <grid2grid/ranks_reordering.hpp>
 
 #include "g2g_oracle.h"
 
@@ -35,6 +40,7 @@
 #include <complex>
 #include <cstring>
 #include <stdexcept>
+#include <unordered_map>
 #include <vector>
 
 using namespace grid2grid;
@@ -227,6 +233,62 @@ int ref_plan_rank(const orc_layout *init, const orc_layout *fin, int dtype, char
         return run<T>(1, init, fin, &trans, nullptr, nullptr, rank, side, counts, dspls, pack_out,
                       pack_cap, total, nullptr);
     });
+}
+
+// transform.cpp:9-39 (communication_volume): the loop of
+// utils.hpp:91-139 (rank_to_comm_vol_for_block) restated on the reference's
+// grid_cover::decompose_block and assigned_grid2D::owner / rows_interval /
+// cols_interval, summed into the reference's comm_volume (edge_t keys), then
+// written out densely (vol[a*n+b], a <= b).
+int ref_comm_volume(const orc_layout *init, const orc_layout *fin, int64_t *vol) {
+    try {
+        assigned_grid2D gi = make_grid(init), gf = make_grid(fin);
+        grid_cover cover(gi.grid(), gf.grid());
+        std::unordered_map<edge_t, int> weights;
+        for (int i = 0; i < gi.grid().n_rows; ++i) {
+            for (int j = 0; j < gi.grid().n_cols; ++j) {
+                block_cover bc = cover.decompose_block(block_coordinates{i, j});
+                interval rows = gi.rows_interval(i), cols = gi.cols_interval(j);
+                std::unordered_map<int, int> per_rank;
+                int row_start = rows.start;
+                for (int a = bc.rows_cover.start_index; a < bc.rows_cover.end_index; ++a) {
+                    int row_end = std::min(gf.grid().rows_split[a + 1], rows.end);
+                    int col_start = cols.start;
+                    for (int b = bc.cols_cover.start_index; b < bc.cols_cover.end_index; ++b) {
+                        int col_end = std::min(gf.grid().cols_split[b + 1], cols.end);
+                        int size = (row_end - row_start) * (col_end - col_start);
+                        if (size) per_rank[gf.owner(a, b)] += size;
+                        col_start = col_end;
+                    }
+                    row_start = row_end;
+                }
+                const int rank = gi.owner(i, j);
+                for (const auto &kv : per_rank)
+                    weights[edge_t{rank, kv.first}.sorted()] += kv.second;
+            }
+        }
+        comm_volume cv(std::move(weights));
+        const int n = std::max(init->n_ranks, fin->n_ranks);
+        std::memset(vol, 0, sizeof(int64_t) * (size_t)n * n);
+        for (const auto &kv : cv.volume) vol[(size_t)kv.first.src * n + kv.first.dest] = kv.second;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// ranks_reordering.cpp:4-61, the reference's own optimal_reordering, on a
+// comm_volume holding the nonzero entries of the dense upper triangle.
+int ref_optimal_reordering(const int64_t *vol, int n, int32_t *perm, int *reordered) {
+    comm_volume cv;
+    for (int a = 0; a < n; ++a)
+        for (int b = a; b < n; ++b)
+            if (vol[(size_t)a * n + b] > 0) cv.volume[edge_t{a, b}] = (int)vol[(size_t)a * n + b];
+    bool r = false;
+    std::vector<int> p = optimal_reordering(cv, n, r);
+    for (int a = 0; a < n; ++a) perm[a] = p[a];
+    *reordered = r ? 1 : 0;
+    return 0;
 }
 
 } // extern "C"

@@ -193,6 +193,43 @@ static bool transformer_case() {
     return ok;
 }
 
+// communication_volume against a count over elements, and optimal_reordering:
+// relabelling the final grid's ranks with the returned permutation lowers the
+// remote volume by exactly the adjusted weights of the matched pairs
+// (ranks_reordering.cpp:17-34).
+static bool volume_case() {
+    scalapack::data_layout a({30, 20}, {4, 3}, {2, 2}, scalapack::column_major);
+    scalapack::data_layout b({30, 20}, {7, 5}, {2, 2}, scalapack::row_major);
+    std::vector<double> dummy(scalapack::local_size(0, a) + scalapack::local_size(0, b));
+    assigned_grid2D gi = get_scalapack_grid<double>(a, dummy.data(), 0).grid;
+    assigned_grid2D gf = get_scalapack_grid<double>(b, dummy.data(), 0).grid;
+    auto owner_at = [](const assigned_grid2D &g, int i, int j) {
+        int bi = 0, bj = 0;
+        while (g.grid().rows_split[bi + 1] <= i) ++bi;
+        while (g.grid().cols_split[bj + 1] <= j) ++bj;
+        return g.owner(bi, bj);
+    };
+    comm_volume v0 = communication_volume(gi, gf);
+    comm_volume brute;
+    for (int i = 0; i < 30; ++i)
+        for (int j = 0; j < 20; ++j) brute.volume[edge_t{owner_at(gi, i, j), owner_at(gf, i, j)}.sorted()] += 1;
+    bool ok = v0.volume == brute.volume;
+    bool reordered = false;
+    std::vector<int> perm = optimal_reordering(v0, 4, reordered);
+    long gain = 0;
+    for (int r = 0; r < 4; ++r) {
+        const int s = perm[r];
+        ok &= perm[s] == r;
+        if (r < s) gain += (long)v0.volume[edge_t{r, s}] - v0.volume[edge_t{r, r}] - v0.volume[edge_t{s, s}];
+    }
+    gf.reorder_ranks(perm);
+    comm_volume v1 = communication_volume(gi, gf);
+    ok &= reordered && gain > 0 && (long)v1.total_volume() == (long)v0.total_volume() - gain;
+    std::printf("communication_volume + optimal_reordering (remote %zu -> %zu): %s\n", v0.total_volume(),
+                v1.total_volume(), ok ? "correct" : "NOT correct");
+    return ok;
+}
+
 int main(int argc, char **argv) {
     bool ok = submatrix_kats();
     if (argc == 2 && std::string(argv[1]) == "--kat-only") return !ok;  // host-only part
@@ -207,6 +244,7 @@ int main(int argc, char **argv) {
     for (const auto &c : cases) ok &= run_case(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], 'N');
     ok &= run_case(64, 48, 8, 8, 5, 7, 2, 2, 'T');
     ok &= transformer_case();
+    ok &= volume_case();
     std::printf("%s\n", ok ? "ALL CORRECT" : "FAILURES");
     return !ok;
 }

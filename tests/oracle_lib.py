@@ -51,6 +51,8 @@ def oracle():
         lib.orc_fill_layout.argtypes = [vp, i, ctypes.c_uint64]
         lib.orc_check_layout.restype = i64
         lib.orc_check_layout.argtypes = [vp, i, ctypes.c_uint64, ctypes.c_char, i64]
+        lib.orc_comm_volume.argtypes = [vp, vp, p64]
+        lib.orc_optimal_reordering.argtypes = [p64, i, p32, ctypes.POINTER(ctypes.c_int)]
         _oracle = lib
     return _oracle
 
@@ -65,6 +67,8 @@ def ref():
         lib.ref_decomp_cover.argtypes = [p32, i, p32, i, p32]
         lib.ref_transform.argtypes = [i, vp, vp, i, ctypes.c_char_p, vp, vp, ctypes.POINTER(ctypes.c_double)]
         lib.ref_plan_rank.argtypes = [vp, vp, i, ctypes.c_char, i, i, p64, p64, vp, i64, p64]
+        lib.ref_comm_volume.argtypes = [vp, vp, p64]
+        lib.ref_optimal_reordering.argtypes = [p64, i, p32, ctypes.POINTER(ctypes.c_int)]
         _ref = lib
     return _ref
 
@@ -219,3 +223,29 @@ def value(dtype, seed, i, j, n_cols):
     buf = np.zeros(16, dtype=np.uint8)
     oracle().orc_value(dtype, seed, i, j, n_cols, buf.ctypes.data)
     return buf[:ES[dtype]].copy()
+
+
+def comm_volume(lib_fn, init_layout, fin_layout):
+    """Dense upper-triangular volume matrix from orc_/ref_comm_volume."""
+    n = max(init_layout.n_ranks, fin_layout.n_ranks)
+    di, df = init_layout.descriptor(0, 1), fin_layout.descriptor(0, 1)
+    vol = np.zeros((n, n), dtype=np.int64)
+    rc = lib_fn(ctypes.addressof(di.struct), ctypes.addressof(df.struct), i64p(vol))
+    assert rc == 0, "comm_volume failed"
+    return vol
+
+
+def volume_dict(vol):
+    a, b = np.nonzero(vol)
+    return {(int(x), int(y)): int(vol[x, y]) for x, y in zip(a, b)}
+
+
+def optimal_reordering(lib_fn, vol):
+    """(permutation, reordered) from orc_/ref_optimal_reordering on a dense volume."""
+    vol = np.ascontiguousarray(vol, dtype=np.int64)
+    n = vol.shape[0]
+    perm = np.zeros(n, dtype=np.int32)
+    r = ctypes.c_int()
+    rc = lib_fn(i64p(vol), n, i32p(perm), ctypes.byref(r))
+    assert rc == 0, "optimal_reordering failed"
+    return [int(x) for x in perm], bool(r.value)

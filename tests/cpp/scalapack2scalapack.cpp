@@ -151,6 +151,56 @@ static bool submatrix_kats() {
     return ok;
 }
 
+// The ScaLAPACK helpers (scalapack_layout.hpp:264-386 of the reference):
+// global <-> local coordinates are inverse to each other and agree with the
+// block placement of get_scalapack_grid; initialize_locally + validate round
+// trip; rank_to_grid/rank_from_grid with a process-grid origin are inverse.
+static bool helpers_case() {
+    bool ok = true;
+    const int m = 23, n = 17;
+    for (scalapack::ordering ord : {scalapack::column_major, scalapack::row_major}) {
+        scalapack::data_layout l({m, n}, {4, 3}, {2, 3}, ord);
+        scalapack::matrix_grid mg(l.matrix_dimension, l.block_dimension);
+        for (int gi = 0; gi < m; ++gi)
+            for (int gj = 0; gj < n; ++gj) {
+                scalapack::local_grid_coord lc = scalapack::local_coordinates(mg, l.rank_grid, {gi, gj});
+                scalapack::elem_grid_coord g = scalapack::global_coordinates(mg, l.rank_grid, lc);
+                ok &= g.row == gi && g.col == gj;
+                // the owner's get_scalapack_grid block holds (gi, gj) at the same buffer offset
+                const int rank = scalapack::rank_from_grid(lc.rank_coord, l.rank_grid, ord);
+                std::vector<double> buf(scalapack::local_size(rank, l));
+                grid_layout<double> lay = get_scalapack_grid<double>(l, buf.data(), rank);
+                scalapack::local_blocks lb = scalapack::get_local_blocks(mg, l.rank_grid, lc.rank_coord);
+                bool found = false;
+                for (int k = 0; k < lay.blocks.num_blocks(); ++k) {
+                    const block<double> &b = lay.blocks.get_block(k);
+                    if (gi < b.rows_interval.start || gi >= b.rows_interval.end || gj < b.cols_interval.start ||
+                        gj >= b.cols_interval.end)
+                        continue;
+                    const double *p = b.data + (gj - b.cols_interval.start) * b.stride + (gi - b.rows_interval.start);
+                    found = p == buf.data() + (size_t)lc.el_coord.col * lb.stride + lc.el_coord.row;
+                }
+                ok &= found;
+            }
+        for (int r = 0; r < 6; ++r) {
+            std::vector<double> buf(scalapack::local_size(r, l));
+            scalapack::initialize_locally(buf.data(), cantor, r, l);
+            ok &= scalapack::validate(cantor, buf, r, l);
+            scalapack::rank_grid_coord src{1, 2};
+            scalapack::rank_grid_coord c = scalapack::rank_to_grid(r, l.rank_grid, ord, src);
+            ok &= scalapack::rank_from_grid(c, l.rank_grid, ord, src) == r;
+        }
+    }
+    // the reference's row test is gi > m: the padding row gi == m of a
+    // partial last block maps to (m, gj), one row further is (-1, -1)
+    scalapack::matrix_grid mg({10, 4}, {4, 4});
+    scalapack::elem_grid_coord a = scalapack::global_coordinates(mg, {1, 1}, {{10, 1}, {0, 0}});
+    scalapack::elem_grid_coord b = scalapack::global_coordinates(mg, {1, 1}, {{11, 1}, {0, 0}});
+    ok &= a.row == 10 && a.col == 1 && b.row == -1 && b.col == -1;
+    std::printf("scalapack helpers: %s\n", ok ? "SUCCESS" : "FAIL");
+    return ok;
+}
+
 // transformer<T>: several pairs, mixed 'N'/'T', one launch
 static bool transformer_case() {
     const int P = 4;
@@ -232,6 +282,7 @@ static bool volume_case() {
 
 int main(int argc, char **argv) {
     bool ok = submatrix_kats();
+    ok &= helpers_case();
     if (argc == 2 && std::string(argv[1]) == "--kat-only") return !ok;  // host-only part
     if (argc == 9) {
         int v[8];

@@ -178,14 +178,44 @@ __device__ __forceinline__ void st_a4(gbyte *p, u32x4 v) {
         *q = w;
 }
 
+// Element e of a 16-byte vector holding 16/ES elements of ES bytes.
+template <int ES>
+__device__ __forceinline__ typename bits_t<ES>::type vec_elem(const u32x4 &v, int e) {
+    if constexpr (ES == 4)
+        return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+    else if constexpr (ES == 8)
+        return e == 0 ? u32x2{v.x, v.y} : u32x2{v.z, v.w};
+    else
+        return v;
+}
+template <int ES>
+__device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::type x) {
+    if constexpr (ES == 4) {
+        if (e == 0) v.x = x;
+        else if (e == 1) v.y = x;
+        else if (e == 2) v.z = x;
+        else v.w = x;
+    } else if constexpr (ES == 8) {
+        if (e == 0) {
+            v.x = x.x;
+            v.y = x.y;
+        } else {
+            v.z = x.x;
+            v.w = x.y;
+        }
+    } else {
+        v = x;
+    }
+}
+
 // Runs queued for one flat copy.
 // SEG_VEC: columns of colb bytes starting at s (source, leading dimension
 //   lds_b bytes) and d (destination, ldd_b), each cut into vpc = ceil(colb/16)
 //   16-byte slots, the last one partial when colb is not a multiple of 16.
-// SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece, one
-//   slot per element (vpc = final rows per final column): destination element
-//   (i, c) at d + c*ldd_b + i*es comes from s + i*lds_b + c*es, its imaginary
-//   sign bit flipped for SEG_CONJ.
+// SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece of
+//   ncols final columns, one slot per final row and group of 16/es final columns
+//   (vpc = final rows): destination element (i, c) at d + c*ldd_b + i*es comes
+//   from s + i*lds_b + c*es, its imaginary sign bit flipped for SEG_CONJ.
 // slot0 numbers the segment's first slot in its queue.
 enum : uint32_t { SEG_VEC = 0, SEG_TRANS = 1, SEG_CONJ = 2 };
 struct seg_entry {
@@ -196,7 +226,7 @@ struct seg_entry {
     uint32_t slot0;
     uint32_t vmul;  // division magic for vpc
     uint32_t kind;
-    uint32_t pad;
+    uint32_t ncols;  // element segments: final columns of the piece
 };
 constexpr int kMaxSeg = 64;
 // Transposed pieces with fewer final rows, or narrower final column ranges
@@ -238,10 +268,12 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
             const uint32_t k = base + (uint32_t)u * THREADS;
             tail[u] = 4;
             if (k < total) {
-                while (next0 <= k) {
-                    ++j;
+                if (next0 <= k) {  // walk the slot starts, then load the one entry needed
+                    do {
+                        ++j;
+                        next0 = segs[j + 1].slot0;
+                    } while (next0 <= k);
                     e = segs[j];
-                    next0 = segs[j + 1].slot0;
                 }
                 const uint32_t rel = k - e.slot0;
                 const uint32_t c = magic_div(rel, e.vpc, e.vmul);
@@ -275,39 +307,68 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
     }
 }
 
-// Copy every element slot of the queued SEG_TRANS / SEG_CONJ segments of
-// ES-byte elements: consecutive lanes write consecutive elements of a
-// destination column (and read a strided row of the source).
+// Copy every slot of the queued SEG_TRANS / SEG_CONJ segments of ES-byte
+// elements. A slot is one final row i of a group of VE = 16/ES final columns:
+// one 16-byte read of VE consecutive elements of source storage column i
+// (dword-aligned), written as one element into each of VE destination
+// columns. Consecutive lanes take consecutive rows, so every one of those VE
+// stores is contiguous across the lanes. The last group of a piece may hold
+// fewer than VE columns (seg_entry.ncols = the piece's column count).
 template <int THREADS, bool NT, int ES>
 __device__ __forceinline__ void flat_copy_elems(const seg_entry *segs, uint32_t total) {
     using T = typename bits_t<ES>::type;
-    constexpr int U = ES == 16 ? 4 : 8;
+    constexpr uint32_t VE = 16 / ES;
+    constexpr int U = 4;
     uint32_t j = 0;
     seg_entry e = segs[0];
     uint32_t next0 = segs[1].slot0;
     for (uint32_t base = threadIdx.x; base < total; base += (uint32_t)THREADS * U) {
-        T r[U];
+        u32x4 r[U];
         gbyte *dp[U];
+        uint32_t cnt[U];
+        uint64_t ldd[U];
+        bool conj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t k = base + (uint32_t)u * THREADS;
+            cnt[u] = 0;
             if (k < total) {
-                while (next0 <= k) {
-                    ++j;
+                if (next0 <= k) {  // walk the slot starts, then load the one entry needed
+                    do {
+                        ++j;
+                        next0 = segs[j + 1].slot0;
+                    } while (next0 <= k);
                     e = segs[j];
-                    next0 = segs[j + 1].slot0;
                 }
                 const uint32_t rel = k - e.slot0;
-                const uint32_t c = magic_div(rel, e.vpc, e.vmul);
-                const uint32_t v = rel - c * e.vpc;
+                const uint32_t g = magic_div(rel, e.vpc, e.vmul);
+                const uint32_t v = rel - g * e.vpc;
+                const uint32_t c = g * VE;
+                cnt[u] = min(VE, e.ncols - c);
                 dp[u] = (gbyte *)e.d + c * e.ldd_b + (uint64_t)v * ES;
-                r[u] = ld<NT>((const GLOBAL T *)((const gbyte *)e.s + (uint64_t)v * e.lds_b + (uint64_t)c * ES));
-                if (ES > 4 && e.kind == SEG_CONJ) r[u] = conj_bits(r[u]);
+                ldd[u] = e.ldd_b;
+                conj[u] = ES > 4 && e.kind == SEG_CONJ;
+                const gbyte *sp = (const gbyte *)e.s + (uint64_t)v * e.lds_b + (uint64_t)c * ES;
+                if (cnt[u] == VE) {
+                    r[u] = ld_a4<NT>(sp);
+                } else {  // ragged last group: element by element
+#pragma unroll
+                    for (uint32_t x = 0; x < VE; ++x)
+                        if (x < cnt[u]) vec_set<ES>(r[u], x, *(const GLOBAL T *)(sp + x * ES));
+                }
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (base + (uint32_t)u * THREADS < total) st<NT>((GLOBAL T *)dp[u], r[u]);
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (uint32_t x = 0; x < VE; ++x) {
+                if (x < cnt[u]) {
+                    T val = vec_elem<ES>(r[u], x);
+                    if (conj[u]) val = conj_bits(val);
+                    st<NT>((GLOBAL T *)(dp[u] + x * ldd[u]), val);
+                }
+            }
+        }
     }
 }
 
@@ -339,36 +400,6 @@ __device__ __forceinline__ void copy_cols_scaled(const gbyte *__restrict__ s,
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (base + (uint32_t)u * THREADS < slots) *(GLOBAL T *)(d + dof[u]) = scale_bits(a[u], b[u], rq);
-    }
-}
-
-// Element e of a 16-byte vector holding 16/ES elements of ES bytes.
-template <int ES>
-__device__ __forceinline__ typename bits_t<ES>::type vec_elem(const u32x4 &v, int e) {
-    if constexpr (ES == 4)
-        return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
-    else if constexpr (ES == 8)
-        return e == 0 ? u32x2{v.x, v.y} : u32x2{v.z, v.w};
-    else
-        return v;
-}
-template <int ES>
-__device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::type x) {
-    if constexpr (ES == 4) {
-        if (e == 0) v.x = x;
-        else if (e == 1) v.y = x;
-        else if (e == 2) v.z = x;
-        else v.w = x;
-    } else if constexpr (ES == 8) {
-        if (e == 0) {
-            v.x = x.x;
-            v.y = x.y;
-        } else {
-            v.z = x.x;
-            v.w = x.y;
-        }
-    } else {
-        v = x;
     }
 }
 
@@ -593,7 +624,9 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uns
         uint64_t md = __ballot(kind == PK_DIRECT);
         const uint32_t nseg = (uint32_t)__popcll(mv), ntseg = (uint32_t)__popcll(me);
         if (nseg | ntseg) {
-            const uint32_t slots = (is_v || is_e) ? (c1 - c0) * vpc : 0u;
+            // element segments: one slot per final row and group of 16/es columns
+            const uint32_t groups = is_e ? (c1 - c0 + (16u / es) - 1) / (16u / es) : c1 - c0;
+            const uint32_t slots = (is_v || is_e) ? groups * vpc : 0u;
             const uint32_t sv = wave_scan_u32(is_v ? slots : 0u, lane);
             const uint32_t se = wave_scan_u32(is_e ? slots : 0u, lane);
             const uint32_t total = __shfl(sv, 63, 64), ttotal = __shfl(se, 63, 64);
@@ -611,7 +644,7 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uns
                     e.slot0 = is_v ? sv - slots : se - slots;
                     e.vmul = is_v ? pc.vmul : div_magic(vpc);
                     e.kind = is_v ? SEG_VEC : (flags & REQ_CONJ) ? SEG_CONJ : SEG_TRANS;
-                    e.pad = 0;
+                    e.ncols = c1 - c0;
                 }
                 if (lane == 0) {  // sentinels
                     segs[nseg].slot0 = total;

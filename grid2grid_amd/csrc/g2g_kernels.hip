@@ -436,58 +436,75 @@ __device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restric
     T *tile = reinterpret_cast<T *>(smem);               // tile[c * LDW + r]
     const bool conj = (rq.flags & REQ_CONJ) != 0;
     const bool scale = SCALE && (rq.flags & REQ_SCALE) != 0;
-    for (uint32_t jc = 0; jc < ncols; jc += TR) {
-        for (uint32_t ic = 0; ic < rows; ic += TC) {
+    const uint32_t n_ic = (rows + TC - 1) / TC;
+    const uint32_t n_tiles = n_ic * ((ncols + TR - 1) / TR);
+    // Phase 1 of tile t+1 (global loads into registers) is issued before
+    // phase 2 of tile t (LDS to global), so loads and stores overlap.
+    u32x4 v1[PER];
+    uint32_t cnt1[PER];
+    auto load_tile = [&](uint32_t t) {
+        const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
 #pragma unroll
-            for (uint32_t k = 0; k < PER; ++k) {
-                const uint32_t q = threadIdx.x + k * THREADS;
-                const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
-                const uint32_t i = ic + c, j0 = jc + r0;
-                if (i < rows && j0 < ncols) {
-                    const gbyte *p = s + i * lds_b + (uint64_t)j0 * ES;
-                    T *t = tile + c * LDW + r0;
-                    if (j0 + VE <= ncols) {
-                        const u32x4 v = ld_a4<NT>(p);
+        for (uint32_t k = 0; k < PER; ++k) {
+            const uint32_t q = threadIdx.x + k * THREADS;
+            const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+            const uint32_t i = ic + c, j0 = jc + r0;
+            cnt1[k] = (i < rows && j0 < ncols) ? min(VE, ncols - j0) : 0u;
+            const gbyte *p = s + i * lds_b + (uint64_t)j0 * ES;
+            if (cnt1[k] == VE) {
+                v1[k] = ld_a4<NT>(p);
+            } else {
 #pragma unroll
-                        for (uint32_t e = 0; e < VE; ++e) t[e] = vec_elem<ES>(v, e);
-                    } else {
-                        for (uint32_t e = 0; e < ncols - j0; ++e) t[e] = *(const GLOBAL T *)(p + e * ES);
-                    }
-                }
+                for (uint32_t e = 0; e < VE; ++e)
+                    if (e < cnt1[k]) vec_set<ES>(v1[k], (int)e, *(const GLOBAL T *)(p + e * ES));
             }
-            __syncthreads();
-#pragma unroll
-            for (uint32_t k = 0; k < PER; ++k) {
-                const uint32_t q = threadIdx.x + k * THREADS;
-                const uint32_t r = q >> WL, w = q & (W - 1);
-                const uint32_t j = jc + r, i0 = ic + w * VE;
-                if (j < ncols && i0 < rows) {
-                    gbyte *p = d + j * ldd_b + (uint64_t)i0 * ES;
-                    const T *t = tile + (w * VE) * LDW + r;
-                    if (i0 + VE <= rows) {
-                        u32x4 v, old;
-                        if (scale) old = ld_a4<false>(p);
-#pragma unroll
-                        for (uint32_t e = 0; e < VE; ++e) {
-                            T x = t[e * LDW];
-                            if (conj) x = conj_bits(x);
-                            if (scale) x = scale_bits(x, vec_elem<ES>(old, e), rq);
-                            vec_set<ES>(v, e, x);
-                        }
-                        st_a4<NT>(p, v);
-                    } else {
-                        for (uint32_t e = 0; e < rows - i0; ++e) {
-                            T x = t[e * LDW];
-                            if (conj) x = conj_bits(x);
-                            GLOBAL T *dp = (GLOBAL T *)(p + e * ES);
-                            if (scale) x = scale_bits(x, *dp, rq);
-                            *dp = x;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
         }
+    };
+    if (n_tiles) load_tile(0);
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; ++k) {
+            const uint32_t q = threadIdx.x + k * THREADS;
+            const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+            T *tp = tile + c * LDW + r0;
+#pragma unroll
+            for (uint32_t e = 0; e < VE; ++e)
+                if (e < cnt1[k]) tp[e] = vec_elem<ES>(v1[k], (int)e);
+        }
+        __syncthreads();
+        if (t + 1 < n_tiles) load_tile(t + 1);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; ++k) {
+            const uint32_t q = threadIdx.x + k * THREADS;
+            const uint32_t r = q >> WL, w = q & (W - 1);
+            const uint32_t j = jc + r, i0 = ic + w * VE;
+            if (j < ncols && i0 < rows) {
+                gbyte *p = d + j * ldd_b + (uint64_t)i0 * ES;
+                const T *tp = tile + (w * VE) * LDW + r;
+                if (i0 + VE <= rows) {
+                    u32x4 v, old;
+                    if (scale) old = ld_a4<false>(p);
+#pragma unroll
+                    for (uint32_t e = 0; e < VE; ++e) {
+                        T x = tp[e * LDW];
+                        if (conj) x = conj_bits(x);
+                        if (scale) x = scale_bits(x, vec_elem<ES>(old, e), rq);
+                        vec_set<ES>(v, e, x);
+                    }
+                    st_a4<NT>(p, v);
+                } else {
+                    for (uint32_t e = 0; e < rows - i0; ++e) {
+                        T x = tp[e * LDW];
+                        if (conj) x = conj_bits(x);
+                        GLOBAL T *dp = (GLOBAL T *)(p + e * ES);
+                        if (scale) x = scale_bits(x, *dp, rq);
+                        *dp = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 

@@ -235,7 +235,7 @@ constexpr int kMaxSeg = 64;
 #define G2G_TRANS_ELEM_ROWS 32
 #endif
 #ifndef G2G_TRANS_ELEM_COLB
-#define G2G_TRANS_ELEM_COLB 512
+#define G2G_TRANS_ELEM_COLB 1024
 #endif
 constexpr uint32_t kTransElemRows = G2G_TRANS_ELEM_ROWS;
 constexpr uint32_t kTransElemColBytes = G2G_TRANS_ELEM_COLB;

@@ -83,25 +83,32 @@ WORKLOAD_DESC = {
 }
 
 
-def time_steps(torch, fn, steps, warmup, stream):
+def time_steps(torch, fn, steps, warmup, stream, per_launch=False):
+    """Average ms per step over `steps` back-to-back steps (events only at the
+    two ends of the timed region). With per_launch, a second, separate pass
+    brackets every step with its own event pair on the launch stream and
+    returns those durations too (the kernel's average launch duration)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
-    per = []
-    # one event pair per step on the launch stream: average launch duration
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     start.record(stream)
-    for a, b in evs:
-        a.record(stream)
+    for _ in range(steps):
         fn()
-        b.record(stream)
     end.record(stream)
     torch.cuda.synchronize()
-    for a, b in evs:
-        per.append(a.elapsed_time(b))
-    return start.elapsed_time(end) / steps, per
+    ms = start.elapsed_time(end) / steps
+    per = []
+    if per_launch:
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        per = [a.elapsed_time(b) for a, b in evs]
+    return ms, per
 
 
 def read_traffic(workload, mode):
@@ -367,7 +374,7 @@ def main():
 
     run = {"fused": lambda: plan.transform(stream=stream),
            "staged": lambda: plan.transform_staged(stream=stream)}[args.mode]
-    ms, per = time_steps(torch, run, args.steps, args.warmup, stream)
+    ms, per = time_steps(torch, run, args.steps, args.warmup, stream, per_launch=True)
     a_n_cols = init.n
     bad = g2g.verify_layout(fin, tf.data_ptr(), dtype, 2, trans, a_n_cols, stream)
     nbytes = workloads.algorithmic_bytes(fin, dtype)

@@ -133,6 +133,7 @@ struct wlayout {
 int check_splits(const char *what, const int32_t *s, int32_t n) {
     if (n < 2 || !s) return fail(G2G_ERR_INVALID, "%s: need at least 2 split entries", what);
     if (s[0] < 0) return fail(G2G_ERR_INVALID, "%s: splits must be non-negative", what);
+    if (n == 2 && s[1] == s[0]) return G2G_OK;  // an empty dimension: one empty interval
     for (int32_t i = 1; i < n; ++i)
         if (s[i] <= s[i - 1])
             return fail(G2G_ERR_INVALID, "%s: splits must be strictly increasing (entry %d)", what, i);

@@ -75,6 +75,8 @@ int comm_volume(const g2g_layout *init, const g2g_layout *fin, std::vector<g2g_e
     rc = internal_union_spans(fcs, ics, "columns", uc);
     if (rc != G2G_OK) return rc;
     const int64_t n = (int64_t)ur.size() * (int64_t)uc.size();
+    out.clear();
+    if (n == 0) return G2G_OK;  // an empty matrix exchanges nothing
     if (n > 0x7fffffffll) return internal_fail(G2G_ERR_UNSUPPORTED, "comm_volume: %lld union cells", (long long)n);
     const uint32_t P = (uint32_t)std::max(init->n_ranks, fin->n_ranks);
     const int i_cells = (init->n_rows_split - 1) * (init->n_cols_split - 1);

@@ -70,8 +70,8 @@ typedef struct g2g_block {
 typedef struct g2g_layout {
     int32_t n_rows_split;      /* entries of rows_split = block rows + 1 */
     int32_t n_cols_split;      /* entries of cols_split = block cols + 1 */
-    const int32_t *rows_split; /* strictly increasing */
-    const int32_t *cols_split; /* strictly increasing */
+    const int32_t *rows_split; /* strictly increasing, or {x, x}: an empty dimension */
+    const int32_t *cols_split; /* strictly increasing, or {x, x}: an empty dimension */
     const int32_t *owners;     /* row-major, (n_rows_split-1) x (n_cols_split-1) */
     int32_t n_ranks;           /* virtual ranks of the layout */
     int32_t n_blocks;          /* entries of blocks */

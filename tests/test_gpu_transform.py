@@ -242,3 +242,32 @@ def test_work_item_sizes(tile_bytes, persistent):
         g2g.set_tuning(0, -1)
     np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(got_t, wantT)
+
+
+@pytest.mark.parametrize("shape", [(0, 40), (30, 0), (0, 0)])
+def test_empty_matrix_is_a_no_op(shape):
+    """An empty dimension plans to no pieces; transforms (fused, staged,
+    transposed), volume and counts succeed and write nothing."""
+    m, n = shape
+    init = scalapack_layout(m, n, 4, 3, 2, 2, "C")
+    fin = scalapack_layout(m, n, 5, 7, 2, 2, "C")
+    dtype = g2g.FLOAT64
+    rng = np.random.default_rng(5)
+    ia = _rand_arena(init, dtype, rng) if init.arena_elems else np.zeros(8, dtype=np.uint8)
+    fa = rng.integers(0, 256, size=64, dtype=np.uint8)
+    ti, tf = _dev(ia), _dev(fa)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+    assert plan.info.n_pieces == 0 and plan.info.bytes == 0
+    plan.transform()
+    plan.transform_staged()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(tf.cpu().numpy(), fa)
+    assert not np.any(plan.counts())
+    plan.close()
+    initT = scalapack_layout(n, m, 4, 3, 2, 2, "C")
+    planT = g2g.Plan(initT, ti.data_ptr(), fin, tf.data_ptr(), dtype, "T")
+    planT.transform()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(tf.cpu().numpy(), fa)
+    planT.close()
+    assert g2g.comm_volume(init, fin) == {}

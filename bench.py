@@ -34,7 +34,7 @@ def parse():
                     help="cfg1 | cfg2 | cfg3 | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
-                    help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:persistent "
+                    help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode "
                          "(skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -404,7 +404,7 @@ def main():
             "pieces": int(info.n_pieces),
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
-            "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "persistent")},
+            "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

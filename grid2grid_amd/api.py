@@ -110,8 +110,10 @@ class Plan:
         check(_lib.load().g2g_plan_get_tuning(self.handle, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
-    def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, persistent=0):
-        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(persistent), 0.0)
+    def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, launch_mode=0):
+        """launch_mode: 0 one workgroup per work item, 1 persistent grid, 2 one
+        workgroup per work item with a contiguous range of work items per XCD."""
+        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode), 0.0)
         check(_lib.load().g2g_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, stream=None, reps=3):
@@ -227,8 +229,10 @@ class Server:
             pass
 
 
-def set_tuning(tile_bytes=0, persistent=-1):
-    check(_lib.load().g2g_set_tuning(int(tile_bytes), int(persistent)))
+def set_tuning(tile_bytes=0, launch_mode=-1):
+    """Process-wide override of the work-item size (0 = automatic) and launch
+    mode (-1 = automatic) for plans and launches made afterwards."""
+    check(_lib.load().g2g_set_tuning(int(tile_bytes), int(launch_mode)))
 
 
 def fill_layout(layout, base, dtype, seed, stream=None):

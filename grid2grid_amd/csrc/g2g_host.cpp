@@ -93,7 +93,7 @@ struct dbuf {
 
 // Tuning knobs (g2g_set_tuning).
 std::atomic<int64_t> g_tile_bytes{0};
-std::atomic<int> g_persistent{-1};
+std::atomic<int> g_launch_mode{-1};
 
 int device_cus() {
     static std::once_flag once;
@@ -273,9 +273,18 @@ struct g2g_plan_s {
 namespace {
 
 bool persistent_mode(uint32_t n_tiles) {
-    const int p = g_persistent.load();
-    if (p >= 0) return p != 0;
+    const int p = g_launch_mode.load();
+    if (p >= 0) return p == G2G_LAUNCH_PERSISTENT;
     return n_tiles > (uint32_t)device_cus() * 64u;
+}
+
+void set_launch_mode(launch_cfg &c, int mode) {
+    c.persistent = mode == G2G_LAUNCH_PERSISTENT;
+    c.xcd_ranges = mode == G2G_LAUNCH_DIRECT_XCD;
+}
+
+int launch_mode_of(const launch_cfg &c) {
+    return c.persistent ? G2G_LAUNCH_PERSISTENT : c.xcd_ranges ? G2G_LAUNCH_DIRECT_XCD : G2G_LAUNCH_DIRECT;
 }
 
 int grid_limit() { return device_cus() * 8; }
@@ -339,7 +348,10 @@ void store_tuned(uint64_t key, const tuned_cfg &c) {
 
 launch_cfg default_cfg(uint32_t n_tiles) {
     launch_cfg c;
-    c.persistent = persistent_mode(n_tiles);
+    if (g_launch_mode.load() >= 0)
+        set_launch_mode(c, g_launch_mode.load());
+    else
+        c.persistent = persistent_mode(n_tiles);
     return c;
 }
 
@@ -540,7 +552,7 @@ int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
     a.total_tiles = t.n_tiles;
     a.queue = p->queue.p;
     launch_cfg c = t.cfg;
-    if (g_persistent.load() >= 0) c.persistent = g_persistent.load() != 0;
+    if (g_launch_mode.load() >= 0) set_launch_mode(c, g_launch_mode.load());
     c.features = std::max(request_features(kr.r[0]), request_features(kr.r[1]));
     G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
@@ -559,11 +571,12 @@ int g2g_synchronize(void *stream) {
     return G2G_OK;
 }
 
-int g2g_set_tuning(int64_t tile_bytes, int persistent) {
+int g2g_set_tuning(int64_t tile_bytes, int launch_mode) {
     if (tile_bytes != 0 && (tile_bytes < 4096 || tile_bytes > (1ll << 26)))
         return fail(G2G_ERR_INVALID, "tile_bytes %lld outside [4096, 2^26]", (long long)tile_bytes);
+    if (launch_mode > G2G_LAUNCH_DIRECT_XCD) return fail(G2G_ERR_INVALID, "unknown launch mode %d", launch_mode);
     g_tile_bytes.store(tile_bytes);
-    g_persistent.store(persistent < 0 ? -1 : (persistent ? 1 : 0));
+    g_launch_mode.store(launch_mode < 0 ? -1 : launch_mode);
     return G2G_OK;
 }
 
@@ -657,7 +670,7 @@ int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
     t->threads = p->fused.cfg.threads;
     t->unroll_bytes = p->fused.cfg.unroll_bytes;
     t->nontemporal = p->fused.cfg.nontemporal;
-    t->persistent = p->fused.cfg.persistent;
+    t->launch_mode = launch_mode_of(p->fused.cfg);
     t->ms = p->tuned_ms;
     return G2G_OK;
 }
@@ -665,13 +678,14 @@ int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
 int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
     if (!p || !t) return fail(G2G_ERR_INVALID, "NULL argument");
     if (t->tile_bytes < 4096 || t->tile_bytes > (1ll << 26) || (t->threads != 256 && t->threads != 512) ||
-        (t->unroll_bytes != 64 && t->unroll_bytes != 128))
+        (t->unroll_bytes != 64 && t->unroll_bytes != 128) || t->launch_mode < 0 ||
+        t->launch_mode > G2G_LAUNCH_DIRECT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
     if ((uint64_t)t->tile_bytes != p->fused.tile_bytes) G2G_TRY(retile(p->fused, (uint64_t)t->tile_bytes, nullptr));
     p->fused.cfg.threads = t->threads;
     p->fused.cfg.unroll_bytes = t->unroll_bytes;
     p->fused.cfg.nontemporal = t->nontemporal != 0;
-    p->fused.cfg.persistent = t->persistent != 0;
+    set_launch_mode(p->fused.cfg, t->launch_mode);
     return G2G_OK;
 }
 
@@ -698,12 +712,12 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         for (int th : {256, 512})
             for (int ub : {64, 128})
                 for (int nt = 0; nt < 2; ++nt)
-                    for (int pe = 0; pe < 2; ++pe) {
+                    for (int mode : {G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_DIRECT_XCD}) {
                         launch_cfg c;
                         c.threads = th;
                         c.unroll_bytes = ub;
                         c.nontemporal = nt != 0;
-                        c.persistent = pe != 0;
+                        set_launch_mode(c, mode);
                         c.features = request_features(kr.r[0]);
                         p->fused.cfg = c;
                         gather_args a;
@@ -1074,7 +1088,7 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->threads = b->cfg.threads;
         best->unroll_bytes = b->cfg.unroll_bytes;
         best->nontemporal = b->cfg.nontemporal;
-        best->persistent = 1;
+        best->launch_mode = G2G_LAUNCH_PERSISTENT;
         best->ms = b->tuned_ms;
     }
     return G2G_OK;

@@ -144,6 +144,7 @@ struct gather_args {
     const uint32_t *tile_owner;   // descs[tile_owner[w]] owns work index w
     work_desc one;
     uint32_t total_tiles;
+    uint32_t xcd_ranges;          // direct mode: each XCD takes one contiguous range of work items
     uint32_t *queue;              // persistent-mode work counter (reset by the launcher)
 };
 // Compiled variants of the gather: workgroup size, bytes each thread keeps in
@@ -160,6 +161,7 @@ struct launch_cfg {
     int unroll_bytes = 64;   // 64 or 128
     bool nontemporal = false;
     bool persistent = false;
+    bool xcd_ranges = false;  // direct launches only: contiguous work ranges per XCD
     int features = FEAT_ALL;
 };
 

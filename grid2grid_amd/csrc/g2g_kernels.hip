@@ -716,7 +716,14 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     __shared__ seg_entry segs[kMaxSeg + 1];
     __shared__ seg_entry tsegs[FEAT == FEAT_COPY ? 1 : kMaxSeg + 1];
     if (!PERSIST) {
-        process_work<THREADS, UB, NT, FEAT>(a, blockIdx.x, smem, segs, tsegs);
+        uint32_t w = blockIdx.x;
+        if (a.xcd_ranges) {
+            // workgroups are dealt to the 8 XCDs round-robin: give each XCD
+            // (and its L2) one contiguous range of work items instead
+            const uint32_t n = gridDim.x, x = w & 7u, q = n >> 3, r = n & 7u;
+            w = x * q + min(x, r) + (w >> 3);
+        }
+        process_work<THREADS, UB, NT, FEAT>(a, w, smem, segs, tsegs);
         return;
     }
     __shared__ uint32_t s_t;
@@ -768,8 +775,10 @@ static hipError_t launch_th(const gather_args &a, const launch_cfg &c, int gl, h
     return c.threads >= 512 ? launch_ub<P, 512>(a, c, gl, s) : launch_ub<P, 256>(a, c, gl, s);
 }
 
-hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream) {
-    if (a.total_tiles == 0) return hipSuccess;
+hipError_t launch_gather(const gather_args &a_in, const launch_cfg &c, int cus, hipStream_t stream) {
+    if (a_in.total_tiles == 0) return hipSuccess;
+    gather_args a = a_in;
+    a.xcd_ranges = (!c.persistent && c.xcd_ranges) ? 1u : 0u;
     // persistent grids fill every CU at full occupancy (32 waves per CU)
     const int per_cu = c.threads >= 512 ? 4 : 8;
     const int gl = cus * per_cu;

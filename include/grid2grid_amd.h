@@ -248,8 +248,15 @@ int g2g_server_destroy(g2g_server s);
 /* ------------------------------------------------------------ tuning */
 
 /* Work-item size (bytes) and launch mode used for plans created afterwards:
- * tile_bytes in [4096, 1<<26] (0 = automatic), persistent 0/1 (-1 = automatic). */
-int g2g_set_tuning(int64_t tile_bytes, int persistent);
+ * tile_bytes in [4096, 1<<26] (0 = automatic), launch_mode G2G_LAUNCH_* (-1 = automatic). */
+int g2g_set_tuning(int64_t tile_bytes, int launch_mode);
+
+/* Launch modes of the gather: one workgroup per work item (work items dealt
+ * round-robin over the XCDs by the hardware), a persistent grid pulling work
+ * items from a device counter, or one workgroup per work item with each XCD
+ * taking one contiguous range of work items (neighbouring work items share
+ * that XCD's L2). */
+enum { G2G_LAUNCH_DIRECT = 0, G2G_LAUNCH_PERSISTENT = 1, G2G_LAUNCH_DIRECT_XCD = 2 };
 
 /* Launch configuration of a plan's fused gather. */
 typedef struct g2g_tuning {
@@ -257,7 +264,7 @@ typedef struct g2g_tuning {
     int32_t threads;       /* workgroup size: 256 or 512 */
     int32_t unroll_bytes;  /* bytes each thread keeps in flight: 64 or 128 */
     int32_t nontemporal;   /* 1: non-temporal loads/stores */
-    int32_t persistent;    /* 0: one workgroup per work item; 1: persistent grid + device queue */
+    int32_t launch_mode;   /* G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT or G2G_LAUNCH_DIRECT_XCD */
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
 } g2g_tuning;
 

@@ -222,15 +222,16 @@ def test_repeat_and_reuse_plan():
     plan.close()
 
 
-@pytest.mark.parametrize("tile_bytes,persistent", [(4096, 0), (4096, 1), (65536, 1), (1 << 20, 0)])
-def test_work_item_sizes(tile_bytes, persistent):
+@pytest.mark.parametrize("tile_bytes,launch_mode", [(4096, 0), (4096, 1), (65536, 1), (1 << 20, 0), (4096, 2),
+                                                     (65536, 2)])
+def test_work_item_sizes(tile_bytes, launch_mode):
     """Results do not depend on the work-item size or launch mode."""
     init, fin = _layout_pairs()["arbitrary"]
     dtype = g2g.COMPLEX128
     rng = np.random.default_rng(77)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
     want = _expected(init, ia, fin, fa, dtype)
-    g2g.set_tuning(tile_bytes, persistent)
+    g2g.set_tuning(tile_bytes, launch_mode)
     try:
         got = _run(init, ia, fin, fa, dtype)
         got_t = None

@@ -279,12 +279,13 @@ bool persistent_mode(uint32_t n_tiles) {
 }
 
 void set_launch_mode(launch_cfg &c, int mode) {
-    c.persistent = mode == G2G_LAUNCH_PERSISTENT;
-    c.xcd_ranges = mode == G2G_LAUNCH_DIRECT_XCD;
+    c.persistent = mode == G2G_LAUNCH_PERSISTENT || mode == G2G_LAUNCH_PERSISTENT_XCD;
+    c.xcd_ranges = mode == G2G_LAUNCH_DIRECT_XCD || mode == G2G_LAUNCH_PERSISTENT_XCD;
 }
 
 int launch_mode_of(const launch_cfg &c) {
-    return c.persistent ? G2G_LAUNCH_PERSISTENT : c.xcd_ranges ? G2G_LAUNCH_DIRECT_XCD : G2G_LAUNCH_DIRECT;
+    if (c.persistent) return c.xcd_ranges ? G2G_LAUNCH_PERSISTENT_XCD : G2G_LAUNCH_PERSISTENT;
+    return c.xcd_ranges ? G2G_LAUNCH_DIRECT_XCD : G2G_LAUNCH_DIRECT;
 }
 
 int grid_limit() { return device_cus() * 8; }
@@ -574,7 +575,7 @@ int g2g_synchronize(void *stream) {
 int g2g_set_tuning(int64_t tile_bytes, int launch_mode) {
     if (tile_bytes != 0 && (tile_bytes < 4096 || tile_bytes > (1ll << 26)))
         return fail(G2G_ERR_INVALID, "tile_bytes %lld outside [4096, 2^26]", (long long)tile_bytes);
-    if (launch_mode > G2G_LAUNCH_DIRECT_XCD) return fail(G2G_ERR_INVALID, "unknown launch mode %d", launch_mode);
+    if (launch_mode > G2G_LAUNCH_PERSISTENT_XCD) return fail(G2G_ERR_INVALID, "unknown launch mode %d", launch_mode);
     g_tile_bytes.store(tile_bytes);
     g_launch_mode.store(launch_mode < 0 ? -1 : launch_mode);
     return G2G_OK;
@@ -629,7 +630,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     G2G_TRY(p->fused.pieces.alloc((size_t)np));
     G2G_TRY(p->fused.pend.alloc((size_t)np));
     G2G_TRY(p->ranks.alloc((size_t)np));
-    G2G_TRY(p->queue.alloc(1));
+    G2G_TRY(p->queue.alloc(kQueueWords));
     G2G_HIP(launch_plan_pieces(d_ur.p, (int32_t)n_ur, d_uc.p, (int32_t)n_uc, d_irs.p, d_ics.p, d_frs.p, d_fcs.p,
                                d_ic.p, wi.nbc, d_fc.p, wf.nbc, es, trans != 'N', p->fused.pieces.p,
                                p->fused.pend.p, p->ranks.p, s));
@@ -679,7 +680,7 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
     if (!p || !t) return fail(G2G_ERR_INVALID, "NULL argument");
     if (t->tile_bytes < 4096 || t->tile_bytes > (1ll << 26) || (t->threads != 256 && t->threads != 512) ||
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) || t->launch_mode < 0 ||
-        t->launch_mode > G2G_LAUNCH_DIRECT_XCD)
+        t->launch_mode > G2G_LAUNCH_PERSISTENT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
     if ((uint64_t)t->tile_bytes != p->fused.tile_bytes) G2G_TRY(retile(p->fused, (uint64_t)t->tile_bytes, nullptr));
     p->fused.cfg.threads = t->threads;
@@ -712,7 +713,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         for (int th : {256, 512})
             for (int ub : {64, 128})
                 for (int nt = 0; nt < 2; ++nt)
-                    for (int mode : {G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_DIRECT_XCD}) {
+                    for (int mode : {G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_DIRECT_XCD,
+                                     G2G_LAUNCH_PERSISTENT_XCD}) {
                         launch_cfg c;
                         c.threads = th;
                         c.unroll_bytes = ub;
@@ -1006,8 +1008,10 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
     uint32_t tiles = 0;
     G2G_TRY(internal_descs(n, plans, alphas, betas, b->host_descs, tiles, b->total_bytes));
     b->plans.assign(plans, plans + n);
-    G2G_TRY(b->queue.alloc(1));
-    b->cfg.persistent = true;
+    G2G_TRY(b->queue.alloc(kQueueWords));
+    // batches always run persistent; g2g_set_tuning may ask for per-XCD ranges
+    set_launch_mode(b->cfg, g_launch_mode.load() == G2G_LAUNCH_PERSISTENT_XCD ? G2G_LAUNCH_PERSISTENT_XCD
+                                                                           : G2G_LAUNCH_PERSISTENT);
     const int64_t forced = g_tile_bytes.load();
     G2G_TRY(batch_retile(b, forced > 0 ? (uint64_t)forced : (uint64_t)131072, s));
     *out = b;
@@ -1056,11 +1060,13 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         if ((rc = batch_retile(b, tb, s)) != G2G_OK) break;
         for (int th : {256, 512})
             for (int ub : {64, 128})
-                for (int nt = 0; nt < 2 && rc == G2G_OK; ++nt) {
+                for (int nt = 0; nt < 2 && rc == G2G_OK; ++nt)
+                for (int mode : {G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_PERSISTENT_XCD}) {
+                    if (rc != G2G_OK) break;
                     b->cfg.threads = th;
                     b->cfg.unroll_bytes = ub;
                     b->cfg.nontemporal = nt != 0;
-                    b->cfg.persistent = true;
+                    set_launch_mode(b->cfg, mode);
                     rc = batch_launch(b, s);  // warm-up
                     hipError_t he = rc == G2G_OK ? hipEventRecord(e0, s) : hipErrorUnknown;
                     for (int r = 0; r < reps && he == hipSuccess && rc == G2G_OK; ++r) rc = batch_launch(b, s);
@@ -1088,7 +1094,7 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->threads = b->cfg.threads;
         best->unroll_bytes = b->cfg.unroll_bytes;
         best->nontemporal = b->cfg.nontemporal;
-        best->launch_mode = G2G_LAUNCH_PERSISTENT;
+        best->launch_mode = launch_mode_of(b->cfg);
         best->ms = b->tuned_ms;
     }
     return G2G_OK;

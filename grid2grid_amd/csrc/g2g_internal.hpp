@@ -12,6 +12,7 @@ struct g2g_plan_s;
 namespace g2g {
 
 constexpr int kThreads = 256;    // threads per workgroup of the planning kernels
+constexpr int kQueueWords = 8;   // work counters of a persistent launch (one per XCD)
 
 enum : uint32_t {
     REQ_TRANS = 1u,  // destination(i, j) = source(j, i)
@@ -144,8 +145,8 @@ struct gather_args {
     const uint32_t *tile_owner;   // descs[tile_owner[w]] owns work index w
     work_desc one;
     uint32_t total_tiles;
-    uint32_t xcd_ranges;          // direct mode: each XCD takes one contiguous range of work items
-    uint32_t *queue;              // persistent-mode work counter (reset by the launcher)
+    uint32_t xcd_ranges;          // each XCD takes one contiguous range of work items
+    uint32_t *queue;              // persistent-mode work counters, kQueueWords of them (reset by the launcher)
 };
 // Compiled variants of the gather: workgroup size, bytes each thread keeps in
 // flight, non-temporal hint, launch mode.
@@ -161,7 +162,7 @@ struct launch_cfg {
     int unroll_bytes = 64;   // 64 or 128
     bool nontemporal = false;
     bool persistent = false;
-    bool xcd_ranges = false;  // direct launches only: contiguous work ranges per XCD
+    bool xcd_ranges = false;  // contiguous work ranges per XCD (persistent: one counter per XCD)
     int features = FEAT_ALL;
 };
 

@@ -727,19 +727,41 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
         return;
     }
     __shared__ uint32_t s_t;
-    for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
-        __syncthreads();
-        const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
-        __syncthreads();
-        if (t >= a.total_tiles) break;
-        process_work<THREADS, UB, NT, FEAT>(a, t, smem, segs, tsegs);
+    if (!a.xcd_ranges) {
+        for (;;) {
+            if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
+            __syncthreads();
+            const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
+            __syncthreads();
+            if (t >= a.total_tiles) break;
+            process_work<THREADS, UB, NT, FEAT>(a, t, smem, segs, tsegs);
+        }
+        return;
+    }
+    // One contiguous range of work items and one counter per XCD (workgroups
+    // are dealt to the 8 XCDs round-robin); a workgroup drains its own XCD's
+    // range, then helps the others in turn, so every range is drained and
+    // every workgroup exits after visiting all eight.
+    const uint32_t n = a.total_tiles, q = n >> 3, r = n & 7u, x0 = blockIdx.x & 7u;
+    for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t x = (x0 + i) & 7u;
+        const uint32_t lo = x * q + min(x, r), len = q + (x < r ? 1u : 0u);
+        for (;;) {
+            if (threadIdx.x == 0) s_t = atomicAdd(a.queue + x, 1u);
+            __syncthreads();
+            const uint32_t c = __builtin_amdgcn_readfirstlane(s_t);
+            __syncthreads();
+            if (c >= len) break;
+            process_work<THREADS, UB, NT, FEAT>(a, lo + c, smem, segs, tsegs);
+        }
     }
 }
 
-// Resets the work counter; a kernel (not a memset) so that it can sit in a
+// Resets the work counters; a kernel (not a memset) so that it can sit in a
 // captured hipGraph.
-__global__ void k_reset_queue(uint32_t *q) { *q = 0; }
+__global__ void k_reset_queue(uint32_t *q) {
+    if (threadIdx.x < (unsigned)kQueueWords) q[threadIdx.x] = 0;
+}
 
 template <bool P, int TH, int UB, bool NT, int F>
 static hipError_t launch_variant(const gather_args &a, int grid_limit, hipStream_t stream) {
@@ -747,7 +769,7 @@ static hipError_t launch_variant(const gather_args &a, int grid_limit, hipStream
         hipLaunchKernelGGL((k_gather<false, TH, UB, NT, F>), dim3(a.total_tiles), dim3(TH), 0, stream, a);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, a.queue);
     const uint32_t grid = std::min<uint32_t>(a.total_tiles, (uint32_t)std::max(1, grid_limit));
     hipLaunchKernelGGL((k_gather<true, TH, UB, NT, F>), dim3(grid), dim3(TH), 0, stream, a);
     return hipGetLastError();
@@ -778,7 +800,7 @@ static hipError_t launch_th(const gather_args &a, const launch_cfg &c, int gl, h
 hipError_t launch_gather(const gather_args &a_in, const launch_cfg &c, int cus, hipStream_t stream) {
     if (a_in.total_tiles == 0) return hipSuccess;
     gather_args a = a_in;
-    a.xcd_ranges = (!c.persistent && c.xcd_ranges) ? 1u : 0u;
+    a.xcd_ranges = c.xcd_ranges ? 1u : 0u;
     // persistent grids fill every CU at full occupancy (32 waves per CU)
     const int per_cu = c.threads >= 512 ? 4 : 8;
     const int gl = cus * per_cu;

@@ -84,7 +84,7 @@ int slot_init(slot &s, int max_batch) {
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&s.h_descs, sizeof(work_desc) * max_batch, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&s.d_descs, sizeof(work_desc) * max_batch) != hipSuccess ||
-        hipMalloc(&s.d_queue, sizeof(uint32_t)) != hipSuccess)
+        hipMalloc(&s.d_queue, sizeof(uint32_t) * kQueueWords) != hipSuccess)
         return internal_fail(G2G_ERR_HIP, "server: stream/event/buffer creation failed");
     return G2G_OK;
 }

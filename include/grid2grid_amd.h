@@ -252,11 +252,16 @@ int g2g_server_destroy(g2g_server s);
 int g2g_set_tuning(int64_t tile_bytes, int launch_mode);
 
 /* Launch modes of the gather: one workgroup per work item (work items dealt
- * round-robin over the XCDs by the hardware), a persistent grid pulling work
- * items from a device counter, or one workgroup per work item with each XCD
- * taking one contiguous range of work items (neighbouring work items share
- * that XCD's L2). */
-enum { G2G_LAUNCH_DIRECT = 0, G2G_LAUNCH_PERSISTENT = 1, G2G_LAUNCH_DIRECT_XCD = 2 };
+ * round-robin over the XCDs by the hardware); a persistent grid pulling work
+ * items from a device counter; and the same two with each XCD taking one
+ * contiguous range of work items (neighbouring work items share that XCD's
+ * L2; persistent: one counter per XCD, idle XCDs then help the others). */
+enum {
+    G2G_LAUNCH_DIRECT = 0,
+    G2G_LAUNCH_PERSISTENT = 1,
+    G2G_LAUNCH_DIRECT_XCD = 2,
+    G2G_LAUNCH_PERSISTENT_XCD = 3
+};
 
 /* Launch configuration of a plan's fused gather. */
 typedef struct g2g_tuning {
@@ -264,7 +269,7 @@ typedef struct g2g_tuning {
     int32_t threads;       /* workgroup size: 256 or 512 */
     int32_t unroll_bytes;  /* bytes each thread keeps in flight: 64 or 128 */
     int32_t nontemporal;   /* 1: non-temporal loads/stores */
-    int32_t launch_mode;   /* G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT or G2G_LAUNCH_DIRECT_XCD */
+    int32_t launch_mode;   /* G2G_LAUNCH_* */
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
 } g2g_tuning;
 

@@ -73,6 +73,33 @@ def test_batch_mixed_equals_alone():
     batch.close()
 
 
+@pytest.mark.parametrize("launch_mode", [1, 3], ids=["one_queue", "queue_per_xcd"])
+def test_batch_launch_modes_and_autotune(launch_mode):
+    """Both persistent launch modes, and whatever the batch autotuner picks,
+    give every request's own result."""
+    cases = _cases(20, 4, scaled_every=6)
+    g2g.set_tuning(0, launch_mode)
+    try:
+        batch = g2g.Batch([c.plan for c in cases], [c.alpha for c in cases], [c.beta for c in cases])
+    finally:
+        g2g.set_tuning(0, -1)
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, c in enumerate(cases):
+        c.check(f"mode {launch_mode} request {k}")
+        c.reset()
+    best = batch.autotune(reps=1)
+    assert best["launch_mode"] in (1, 3)
+    for c in cases:
+        c.reset()
+    torch.cuda.synchronize()
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, c in enumerate(cases):
+        c.check(f"autotuned {best} request {k}")
+    batch.close()
+
+
 def test_batch_graph_replays():
     """The captured graph gives the eager result on every replay, with eager
     work and synchronizations between the replays."""

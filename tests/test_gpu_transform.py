@@ -223,7 +223,7 @@ def test_repeat_and_reuse_plan():
 
 
 @pytest.mark.parametrize("tile_bytes,launch_mode", [(4096, 0), (4096, 1), (65536, 1), (1 << 20, 0), (4096, 2),
-                                                     (65536, 2)])
+                                                     (65536, 2), (4096, 3), (1 << 20, 3)])
 def test_work_item_sizes(tile_bytes, launch_mode):
     """Results do not depend on the work-item size or launch mode."""
     init, fin = _layout_pairs()["arbitrary"]

@@ -232,10 +232,10 @@ constexpr int kMaxSeg = 64;
 // Transposed pieces with fewer final rows, or narrower final column ranges
 // (bytes), than these go to the element queue instead of LDS tiles.
 #ifndef G2G_TRANS_ELEM_ROWS
-#define G2G_TRANS_ELEM_ROWS 32
+#define G2G_TRANS_ELEM_ROWS 64
 #endif
 #ifndef G2G_TRANS_ELEM_COLB
-#define G2G_TRANS_ELEM_COLB 1024
+#define G2G_TRANS_ELEM_COLB 256
 #endif
 constexpr uint32_t kTransElemRows = G2G_TRANS_ELEM_ROWS;
 constexpr uint32_t kTransElemColBytes = G2G_TRANS_ELEM_COLB;

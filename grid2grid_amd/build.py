@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgrid2grid_amd.so")
 
-SOURCES = ["g2g_kernels.hip", "g2g_host.cpp", "g2g_serve.cpp", "g2g_volume.hip"]
+SOURCES = ["g2g_kernels.hip", "g2g_host.cpp", "g2g_serve.cpp", "g2g_rank_pairs.hip"]
 
 HIPCC_FLAGS = [
     "-std=c++17",

@@ -419,15 +419,22 @@ request make_request(int dtype, char trans, const void *alpha, const void *beta,
 int ensure_wire(g2g_plan p, hipStream_t s) {
     if (p->wire_ready) return G2G_OK;
     const int32_t P = p->P;
-    if ((int64_t)P * P * 8 > 64 * 1024)
-        return fail(G2G_ERR_UNSUPPORTED, "wire order needs P*P*8 <= 64 KiB of LDS (P = %d)", P);
+    // per-pair tables are P*P entries (host and device): bounded here
+    if (P > kMaxWireRanks)
+        return fail(G2G_ERR_UNSUPPORTED, "the plain form supports at most %d ranks (P = %d)", kMaxWireRanks, P);
     const int64_t n = p->fused.n;
-    const int64_t n_chunks = std::max<int64_t>(1, (n + 2047) / 2048);
-    dbuf<uint64_t> hist, pc;
-    G2G_TRY(hist.alloc((size_t)P * P * n_chunks));
+    dbuf<uint64_t> pc;
     G2G_TRY(pc.alloc((size_t)P * P));
     G2G_TRY(p->within.alloc((size_t)n));
-    G2G_HIP(launch_pair_offsets(p->ranks.p, p->fused.pieces.p, n, P, hist.p, n_chunks, p->within.p, pc.p, s));
+    if ((int64_t)P * P * 8 <= 64 * 1024) {  // pair table in LDS: multi-split wavefront scan
+        const int64_t n_chunks = std::max<int64_t>(1, (n + 2047) / 2048);
+        dbuf<uint64_t> hist;
+        G2G_TRY(hist.alloc((size_t)P * P * n_chunks));
+        G2G_HIP(launch_pair_offsets(p->ranks.p, p->fused.pieces.p, n, P, hist.p, n_chunks, p->within.p, pc.p,
+                                    s));
+    } else {
+        G2G_TRY(launch_pair_offsets_sorted(p->ranks.p, p->fused.pieces.p, n, P, p->within.p, pc.p, s));
+    }
     p->counts.assign((size_t)P * P, 0);
     G2G_HIP(hipMemcpyAsync(p->counts.data(), pc.p, sizeof(uint64_t) * P * P, hipMemcpyDeviceToHost, s));
     G2G_HIP(hipStreamSynchronize(s));

@@ -13,6 +13,7 @@ namespace g2g {
 
 constexpr int kThreads = 256;    // threads per workgroup of the planning kernels
 constexpr int kQueueWords = 8;   // work counters of a persistent launch (one per XCD)
+constexpr int kMaxWireRanks = 4096;  // ranks of the plain form (P*P pair tables)
 
 enum : uint32_t {
     REQ_TRANS = 1u,  // destination(i, j) = source(j, i)
@@ -112,6 +113,11 @@ hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_byt
 // (= the reference's (cols interval, rows interval) order); totals per pair.
 hipError_t launch_pair_offsets(const uint32_t *ranks, const piece *pieces, int64_t n,
                                int32_t n_ranks, uint64_t *hist, int64_t n_chunks,
+                               uint64_t *within, uint64_t *pair_counts, hipStream_t stream);
+
+// The same by a stable radix sort of the pieces by pair, for rank counts whose
+// P*P pair table does not fit in LDS. Returns a g2g_status; synchronous.
+int launch_pair_offsets_sorted(const uint32_t *ranks, const piece *pieces, int64_t n, int32_t P,
                                uint64_t *within, uint64_t *pair_counts, hipStream_t stream);
 
 // Stage tables of the plain form from pieces, ranks, within-pair offsets and

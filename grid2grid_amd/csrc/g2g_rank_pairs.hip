@@ -1,6 +1,8 @@
-// Communication volume between the virtual ranks of two grids, and the greedy
-// rank relabelling computed from it (g2g_comm_volume, g2g_optimal_reordering in
-// include/grid2grid_amd.h).
+// Per rank-pair work done by sorting on the device: the communication volume
+// between the virtual ranks of two grids and the greedy rank relabelling
+// computed from it (g2g_comm_volume, g2g_optimal_reordering in
+// include/grid2grid_amd.h), and the wire order of the plain form for rank
+// counts whose pair table does not fit in LDS (launch_pair_offsets_sorted).
 //
 // The reference walks every initial block, finds the final cells it overlaps
 // through the grid cover and adds the overlap sizes to a hash map keyed by the
@@ -134,7 +136,102 @@ struct wedge {
     int64_t w;
 };
 
+// ---- wire order by sorting (many ranks)
+
+constexpr uint32_t kLocalKey = 0xffffffffu;  // pieces whose source and destination rank agree
+
+// keys[p] = source * P + destination (kLocalKey for local pieces), idx[p] = p.
+__global__ __launch_bounds__(256) void k_wire_keys(const uint32_t *ranks, int64_t n, uint32_t P, uint32_t *keys,
+                                                   uint32_t *idx) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t r = ranks[p], s = r >> 16, d = r & 0xffffu;
+    keys[p] = s == d ? kLocalKey : s * P + d;
+    idx[p] = (uint32_t)p;
+}
+
+// In sorted order: element count of each piece (0 for local ones) and, at the
+// first piece of every key, its own position (0 elsewhere; a max-scan turns
+// this into each piece's segment start).
+__global__ __launch_bounds__(256) void k_wire_sizes(const uint32_t *skeys, const uint32_t *sidx, const piece *pieces,
+                                                    int64_t n, uint64_t *sz, uint32_t *head) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const piece pc = pieces[sidx[i]];
+    sz[i] = skeys[i] == kLocalKey ? 0ull : (uint64_t)pc.rows * pc.cols;
+    head[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? (uint32_t)i : 0u;
+}
+
+// within[p]: offset of piece p in its pair's segment (~0 for local pieces);
+// pair_counts[key] at the last piece of each key's segment.
+__global__ __launch_bounds__(256) void k_wire_within(const uint32_t *skeys, const uint32_t *sidx, const uint64_t *sz,
+                                                     const uint64_t *ex, const uint32_t *start, int64_t n,
+                                                     uint64_t *within, uint64_t *pair_counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = skeys[i];
+    if (key == kLocalKey) {
+        within[sidx[i]] = ~0ull;
+        return;
+    }
+    const uint64_t off = ex[i] - ex[start[i]];
+    within[sidx[i]] = off;
+    if (i + 1 == n || skeys[i + 1] != key) pair_counts[key] = off + sz[i];
+}
+
+__global__ __launch_bounds__(256) void k_zero_u64(uint64_t *x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
 }  // namespace
+
+namespace g2g {
+
+// Same results as launch_pair_offsets (reference communication_data.cpp:51-99
+// with the message order of utils.hpp:141-161): a stable radix sort of the
+// pieces by (source, destination) keeps, inside every pair, the pieces in
+// table order, which is the reference's (columns, rows) order; an exclusive
+// scan of the sorted piece sizes, less the scan at each pair's first piece,
+// gives the offsets. Synchronous on `s` (scratch is freed on return).
+int launch_pair_offsets_sorted(const uint32_t *ranks, const piece *pieces, int64_t n, int32_t P,
+                               uint64_t *within, uint64_t *pair_counts, hipStream_t s) {
+    const int64_t K = (int64_t)P * P;
+    if (K >= (int64_t)kLocalKey) return internal_fail(G2G_ERR_UNSUPPORTED, "wire order: %d ranks", P);
+    if (n > 0x7fffffffll) return internal_fail(G2G_ERR_UNSUPPORTED, "wire order: %lld pieces", (long long)n);
+    const unsigned gk = (unsigned)((K + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u64, dim3(gk), dim3(256), 0, s, pair_counts, K);
+    VOL_HIP(hipGetLastError());
+    if (n == 0) return G2G_OK;
+    scratch_set sc;
+    uint32_t *keys = sc.get<uint32_t>(n), *skeys = sc.get<uint32_t>(n), *idx = sc.get<uint32_t>(n),
+             *sidx = sc.get<uint32_t>(n), *head = sc.get<uint32_t>(n), *start = sc.get<uint32_t>(n);
+    uint64_t *sz = sc.get<uint64_t>(n), *ex = sc.get<uint64_t>(n);
+    if (!keys || !skeys || !idx || !sidx || !head || !start || !sz || !ex)
+        return internal_fail(G2G_ERR_NOMEM, "wire order: device allocation failed");
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    const int ni = (int)n;
+    hipLaunchKernelGGL(k_wire_keys, dim3(gn), dim3(256), 0, s, ranks, n, (uint32_t)P, keys, idx);
+    VOL_HIP(hipGetLastError());
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    VOL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, keys, skeys, idx, sidx, ni, 0, 32, s));
+    VOL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, sz, ex, ni, s));
+    VOL_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b3, head, start, hipcub::Max(), ni, s));
+    void *tmp = sc.get<unsigned char>(std::max(b1, std::max(b2, b3)));
+    if (!tmp) return internal_fail(G2G_ERR_NOMEM, "wire order: device allocation failed");
+    VOL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b1, keys, skeys, idx, sidx, ni, 0, 32, s));
+    hipLaunchKernelGGL(k_wire_sizes, dim3(gn), dim3(256), 0, s, skeys, sidx, pieces, n, sz, head);
+    VOL_HIP(hipGetLastError());
+    VOL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b2, sz, ex, ni, s));
+    VOL_HIP(hipcub::DeviceScan::InclusiveScan(tmp, b3, head, start, hipcub::Max(), ni, s));
+    hipLaunchKernelGGL(k_wire_within, dim3(gn), dim3(256), 0, s, skeys, sidx, sz, ex, start, n, within,
+                       pair_counts);
+    VOL_HIP(hipGetLastError());
+    VOL_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+}  // namespace g2g
 
 extern "C" {
 

@@ -141,7 +141,8 @@ int g2g_transform(g2g_plan plan, const void *alpha, const void *beta, void *stre
  * (communication_data.cpp:224-252), exchange each (source, destination)
  * segment between virtual-rank buffers (transform.cpp:53-87), then unpack
  * (communication_data.cpp:157-173). Staging buffers are allocated by the
- * plan on first use. Asynchronous on `stream`. */
+ * plan on first use. Up to 4096 ranks (the per-pair tables hold P*P
+ * entries). Asynchronous on `stream`. */
 int g2g_transform_staged(g2g_plan plan, const void *alpha, const void *beta, void *stream);
 
 /* Per-rank send counts of the plan: counts[s * P + d] = elements rank s sends

@@ -39,6 +39,10 @@ def _layout_pairs():
                       arbitrary_layout(rng, 300, 260, 5, 8, 6, lo=10, hi=80, pad=(0, 9), gap=(0, 33))),
         "single_rank": (scalapack_layout(64, 48, 64, 48, 1, 1), scalapack_layout(64, 48, 16, 16, 1, 1)),
         "tall_thin": (scalapack_layout(1000, 3, 17, 1, 4, 1), scalapack_layout(1000, 3, 300, 2, 2, 2)),
+        # more ranks than the LDS pair table holds: the plain form's wire
+        # order comes from a radix sort by rank pair
+        "many_ranks": (arbitrary_layout(rng, 240, 180, 16, 12, 130, lo=5, hi=25, pad=(0, 5), gap=(0, 9)),
+                       arbitrary_layout(rng, 240, 180, 13, 15, 100, lo=5, hi=25, pad=(0, 5), gap=(0, 9))),
     }
 
 
@@ -139,7 +143,7 @@ def test_scaled_bit_exact(dtype, trans, staged):
         np.testing.assert_array_equal(got, want, err_msg=name)
 
 
-@pytest.mark.parametrize("name", ["bc_small", "bc_grid_change", "arbitrary"])
+@pytest.mark.parametrize("name", ["bc_small", "bc_grid_change", "arbitrary", "many_ranks"])
 def test_wire_order_matches_oracle(name):
     """Counts per (source, destination) and every piece's offset in the send
     and receive buffers equal the reference's communication_data (via the

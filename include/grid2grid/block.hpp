@@ -35,6 +35,18 @@ struct block_range {
     block_range intersection(const block_range &o) const {
         return {rows_interval.intersection(o.rows_interval), cols_interval.intersection(o.cols_interval)};
     }
+    // As in the reference (block.cpp:35-51), including its column test, which
+    // compares cols_interval.end <= range.cols_interval.start twice.
+    bool outside_of(const block_range &range) const {
+        return (rows_interval.end <= range.rows_interval.start ||
+                rows_interval.start >= range.rows_interval.end) &&
+               (cols_interval.end <= range.cols_interval.start || cols_interval.end <= range.cols_interval.start);
+    }
+    bool inside(const block_range &range) const {
+        return range.rows_interval.start < rows_interval.start && range.rows_interval.end > rows_interval.end &&
+               range.cols_interval.start < cols_interval.start && range.cols_interval.end > cols_interval.end;
+    }
+    bool intersects(const block_range &range) const { return !outside_of(range) && !inside(range); }
     bool non_empty() const { return rows_interval.non_empty() && cols_interval.non_empty(); }
     bool empty() const { return rows_interval.empty() || cols_interval.empty(); }
     bool operator==(const block_range &o) const {

@@ -152,7 +152,7 @@ def cpu_baseline(budget_s):
         n += 1
         if n > 2:  # two warm-ups
             times.append(dt)
-        if (time.time() > t_end and len(times) >= 3) or len(times) >= 50:
+        if (time.time() > t_end and len(times) >= 3) or len(times) >= 200:
             break
     med = statistics.median(times)
     return {

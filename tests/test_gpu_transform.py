@@ -276,3 +276,31 @@ def test_empty_matrix_is_a_no_op(shape):
     np.testing.assert_array_equal(tf.cpu().numpy(), fa)
     planT.close()
     assert g2g.comm_volume(init, fin) == {}
+
+
+def test_random_requests_both_forms():
+    """A sweep of random block-cyclic requests (sizes 8..300, blocks 16..256,
+    grids up to 4x4, every type, 'N'/'T'/'C', some scaled), fused and staged,
+    each against the oracle on random bytes."""
+    from grid2grid_amd import workloads
+    rng = np.random.default_rng(2024)
+    for k in range(40):
+        init, fin, dtype, trans = workloads.random_request(rng, dtypes=DTYPES, lo=8, hi=300, p_trans=0.5)
+        if trans == "T" and dtype in (g2g.COMPLEX64, g2g.COMPLEX128) and rng.random() < 0.5:
+            trans = "C"
+        staged = bool(k % 2)
+        alpha = beta = None
+        if k % 5 == 0:
+            alpha, beta = SCALARS[dtype]
+            es = g2g.DTYPE_SIZE[dtype]
+            ia = np.zeros(init.arena_elems * es, dtype=np.uint8)
+            fa = np.zeros(fin.arena_elems * es, dtype=np.uint8)
+            O.fill(init, ia, dtype, 100 + k)
+            O.fill(fin, fa, dtype, 200 + k)
+        else:
+            ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+        print(f"case {k}: {init.m}x{init.n} dtype {dtype} trans {trans} staged {staged} "
+              f"scaled {alpha is not None}", flush=True)
+        want = _expected(init, ia, fin, fa, dtype, trans, alpha, beta)
+        got = _run(init, ia, fin, fa, dtype, trans, alpha, beta, staged=staged)
+        np.testing.assert_array_equal(got, want, err_msg=f"case {k}")

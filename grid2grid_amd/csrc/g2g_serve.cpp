@@ -145,7 +145,6 @@ int g2g_server_s::launch(slot &sl, std::vector<pending_req> &batch) {
         a.queue = sl.d_queue;
         launch_cfg c;
         c.persistent = true;
-        c.xcd_ranges = true;  // one work range and counter per XCD
         c.features = FEAT_COPY;
         for (const work_desc &d : descs) c.features = std::max(c.features, request_features(d.kr.r[0]));
         if (launch_gather(a, c, internal_grid_limit(), sl.stream) != hipSuccess)

@@ -871,7 +871,7 @@ struct g2g_batch_s {
     std::vector<work_desc> host_descs;  // requests as the batch runs them
     dbuf<work_desc> descs;
     dbuf<uint32_t> first;               // per plan: first piece of each batch work item
-    dbuf<uint32_t> owner;
+    dbuf<work_rec> work;  // per global work index: transform and first piece
     dbuf<uint32_t> queue;
     uint32_t total_tiles = 0;
     uint64_t total_bytes = 0;
@@ -895,7 +895,7 @@ int batch_launch(g2g_batch b, hipStream_t s) {
     gather_args a;
     std::memset(&a, 0, sizeof a);
     a.descs = b->descs.p;
-    a.tile_owner = b->owner.p;
+    a.work = b->work.p;
     a.total_tiles = b->total_tiles;
     a.queue = b->queue.p;
     launch_cfg c = b->cfg;
@@ -921,7 +921,7 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
     if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
     b->total_tiles = (uint32_t)tiles;
     G2G_TRY(b->first.alloc(tiles));
-    G2G_TRY(b->owner.alloc(tiles));
+    G2G_TRY(b->work.alloc(tiles));
     for (size_t k = 0; k < b->plans.size(); ++k) {
         const stage_table &t = b->plans[k]->fused;
         work_desc &d = b->host_descs[k];
@@ -932,7 +932,7 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
         G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, b->first.p + off[k], s));
     }
     G2G_TRY(b->descs.upload(b->host_descs.data(), b->host_descs.size(), s));
-    G2G_HIP(launch_tile_owner(b->descs.p, (int32_t)b->host_descs.size(), b->owner.p, s));
+    G2G_HIP(launch_work_records(b->descs.p, (int32_t)b->host_descs.size(), b->work.p, s));
     G2G_HIP(hipStreamSynchronize(s));
     return G2G_OK;
 }

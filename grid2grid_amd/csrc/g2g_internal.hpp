@@ -144,11 +144,18 @@ struct work_desc {
     kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1] (same element type)
 };
 
-// A launch: one desc passed by value, or a device table of descs with the
-// owner of every global work index (multi-transform launches).
+// Global work index w of a multi-transform launch: descs[desc] owns it and
+// its first piece is first_piece (an index into that desc's piece table).
+struct work_rec {
+    uint32_t desc;
+    uint32_t first_piece;
+};
+
+// A launch: one desc passed by value, or a device table of descs with a
+// record for every global work index (multi-transform launches).
 struct gather_args {
     const work_desc *descs;       // device table, or null: `one` is used
-    const uint32_t *tile_owner;   // descs[tile_owner[w]] owns work index w
+    const work_rec *work;         // per global work index
     work_desc one;
     uint32_t total_tiles;
     uint32_t xcd_ranges;          // each XCD takes one contiguous range of work items
@@ -181,9 +188,9 @@ inline int request_features(const request &r) {
 // `cus` = compute units of the device (persistent grids fill them).
 hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream);
 
-// tile_owner[w] = k for w in [descs[k].tile_base, + n_tiles), k < n.
-hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_owner,
-                             hipStream_t stream);
+// work[w] = {k, descs[k].first_piece[w - tile_base]} for w in
+// [descs[k].tile_base, + n_tiles), k < n.
+hipError_t launch_work_records(const work_desc *descs, int32_t n, work_rec *work, hipStream_t stream);
 
 // ------------------------------------------------ host helpers (g2g_host.cpp)
 

@@ -593,16 +593,17 @@ enum : uint32_t { PK_NONE = 0, PK_VEC = 1, PK_ELEM = 2, PK_DIRECT = 3 };
 // by the whole workgroup. Pieces write disjoint destinations, so their order
 // does not matter.
 template <int THREADS, int UB, bool NT, int FEAT>
-__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, unsigned char *smem,
+__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uint32_t p, unsigned char *smem,
                                              seg_entry *segs, seg_entry *tsegs) {
     const uint64_t lo = (uint64_t)t * a.tile_bytes;
     const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t es = a.kr.r[0].esize;  // both requests of a desc share the element type
     const uint32_t n = a.n_pieces;
-    uint32_t p = a.first_piece[t];
+    // p: the work item's first piece; a lane-0 piece past the work item ends
+    // the loop below without a separate check
     for (;;) {
-        if (p >= n || (p ? a.pend[p - 1] : 0) >= hi) break;
+        if (p >= n) break;
         const uint32_t q = p + lane;
         uint32_t kind = PK_NONE, c0 = 0, c1 = 0, rows = 0, flags = 0, vpc = 0;
         piece pc;
@@ -700,12 +701,14 @@ template <int THREADS, int UB, bool NT, int FEAT>
 __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
                                              unsigned char *smem, seg_entry *segs, seg_entry *tsegs) {
     if (!a.descs) {
-        process_tile<THREADS, UB, NT, FEAT>(a.one, w, smem, segs, tsegs);
+        process_tile<THREADS, UB, NT, FEAT>(a.one, w, a.one.first_piece[w], smem, segs, tsegs);
         return;
     }
-    const uint32_t k = a.tile_owner[w];
-    const work_desc d = a.descs[k];
-    process_tile<THREADS, UB, NT, FEAT>(d, w - d.tile_base, smem, segs, tsegs);
+    // one record gives both the transform and the first piece, so the
+    // descriptor and the pieces are the only loads that wait on it
+    const work_rec r = a.work[w];
+    const work_desc d = a.descs[r.desc];
+    process_tile<THREADS, UB, NT, FEAT>(d, w - d.tile_base, r.first_piece, smem, segs, tsegs);
 }
 
 // Direct mode: one workgroup per work index. Persistent mode: a grid sized to
@@ -728,13 +731,20 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     }
     __shared__ uint32_t s_t;
     if (!a.xcd_ranges) {
-        for (;;) {
-            if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
-            __syncthreads();
-            const uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
-            __syncthreads();
-            if (t >= a.total_tiles) break;
+        // the next work index is claimed before the current one is moved, so
+        // the counter's latency overlaps the work
+        if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
+        __syncthreads();
+        while (t < a.total_tiles) {
+            uint32_t next = 0;
+            if (threadIdx.x == 0) next = atomicAdd(a.queue, 1u);
             process_work<THREADS, UB, NT, FEAT>(a, t, smem, segs, tsegs);
+            if (threadIdx.x == 0) s_t = next;
+            __syncthreads();
+            t = __builtin_amdgcn_readfirstlane(s_t);
+            __syncthreads();
         }
         return;
     }
@@ -746,13 +756,18 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     for (uint32_t i = 0; i < 8; ++i) {
         const uint32_t x = (x0 + i) & 7u;
         const uint32_t lo = x * q + min(x, r), len = q + (x < r ? 1u : 0u);
-        for (;;) {
-            if (threadIdx.x == 0) s_t = atomicAdd(a.queue + x, 1u);
-            __syncthreads();
-            const uint32_t c = __builtin_amdgcn_readfirstlane(s_t);
-            __syncthreads();
-            if (c >= len) break;
+        if (threadIdx.x == 0) s_t = atomicAdd(a.queue + x, 1u);
+        __syncthreads();
+        uint32_t c = __builtin_amdgcn_readfirstlane(s_t);
+        __syncthreads();
+        while (c < len) {
+            uint32_t next = 0;
+            if (threadIdx.x == 0) next = atomicAdd(a.queue + x, 1u);  // overlaps the work below
             process_work<THREADS, UB, NT, FEAT>(a, lo + c, smem, segs, tsegs);
+            if (threadIdx.x == 0) s_t = next;
+            __syncthreads();
+            c = __builtin_amdgcn_readfirstlane(s_t);
+            __syncthreads();
         }
     }
 }
@@ -807,16 +822,16 @@ hipError_t launch_gather(const gather_args &a_in, const launch_cfg &c, int cus, 
     return c.persistent ? launch_th<true>(a, c, gl, stream) : launch_th<false>(a, c, gl, stream);
 }
 
-__global__ void k_tile_owner(const work_desc *descs, uint32_t *owner) {
+__global__ void k_work_records(const work_desc *descs, work_rec *work) {
     const uint32_t k = blockIdx.x;
     const uint32_t base = descs[k].tile_base, n = descs[k].n_tiles;
-    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) owner[base + t] = k;
+    const uint32_t *fp = descs[k].first_piece;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) work[base + t] = work_rec{k, fp[t]};
 }
 
-hipError_t launch_tile_owner(const work_desc *descs, int32_t n, uint32_t *tile_owner,
-                             hipStream_t stream) {
+hipError_t launch_work_records(const work_desc *descs, int32_t n, work_rec *work, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_owner, dim3((unsigned)n), dim3(256), 0, stream, descs, tile_owner);
+    hipLaunchKernelGGL(k_work_records, dim3((unsigned)n), dim3(256), 0, stream, descs, work);
     return hipGetLastError();
 }
 

@@ -41,8 +41,8 @@ struct slot {
     hipEvent_t done = nullptr;
     work_desc *h_descs = nullptr;  // pinned staging of the descriptor table
     work_desc *d_descs = nullptr;
-    uint32_t *d_owner = nullptr;
-    uint32_t owner_cap = 0;
+    work_rec *d_work = nullptr;  // per global work index of a launch
+    uint32_t work_cap = 0;
     uint32_t *d_queue = nullptr;
     std::vector<pending_req> reqs;
     bool busy = false;
@@ -91,7 +91,7 @@ int slot_init(slot &s, int max_batch) {
 
 void slot_free(slot &s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
-    if (s.d_owner) (void)hipFree(s.d_owner);
+    if (s.d_work) (void)hipFree(s.d_work);
     if (s.d_queue) (void)hipFree(s.d_queue);
     if (s.d_descs) (void)hipFree(s.d_descs);
     if (s.h_descs) (void)hipHostFree(s.h_descs);
@@ -123,24 +123,24 @@ int g2g_server_s::launch(slot &sl, std::vector<pending_req> &batch) {
         uint64_t total_bytes = 0;
         int rc = internal_descs(n, plans.data(), al.data(), be.data(), descs, total_tiles, total_bytes);
         if (rc != G2G_OK) return rc;
-        if (total_tiles > sl.owner_cap) {
-            if (sl.d_owner) (void)hipFree(sl.d_owner);
-            sl.d_owner = nullptr;
+        if (total_tiles > sl.work_cap) {
+            if (sl.d_work) (void)hipFree(sl.d_work);
+            sl.d_work = nullptr;
             const uint32_t cap = std::max<uint32_t>(total_tiles, 1u << 16);
-            if (hipMalloc(&sl.d_owner, sizeof(uint32_t) * cap) != hipSuccess)
+            if (hipMalloc(&sl.d_work, sizeof(work_rec) * cap) != hipSuccess)
                 return internal_fail(G2G_ERR_NOMEM, "server: work table allocation failed");
-            sl.owner_cap = cap;
+            sl.work_cap = cap;
         }
         std::memcpy(sl.h_descs, descs.data(), sizeof(work_desc) * n);
         if (hipMemcpyAsync(sl.d_descs, sl.h_descs, sizeof(work_desc) * n, hipMemcpyHostToDevice, sl.stream) !=
             hipSuccess)
             return internal_fail(G2G_ERR_HIP, "server: descriptor upload failed");
-        if (launch_tile_owner(sl.d_descs, n, sl.d_owner, sl.stream) != hipSuccess)
+        if (launch_work_records(sl.d_descs, n, sl.d_work, sl.stream) != hipSuccess)
             return internal_fail(G2G_ERR_HIP, "server: work table launch failed");
         gather_args a;
         std::memset(&a, 0, sizeof a);
         a.descs = sl.d_descs;
-        a.tile_owner = sl.d_owner;
+        a.work = sl.d_work;
         a.total_tiles = total_tiles;
         a.queue = sl.d_queue;
         launch_cfg c;

@@ -26,6 +26,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # packed-FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # atomics stay as written: the gather's work counter is read after the
+    # work it overlaps, not right after it is issued
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wall",
     "-Wno-unused-function",
 ]

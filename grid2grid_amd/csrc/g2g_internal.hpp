@@ -144,12 +144,20 @@ struct work_desc {
     kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1] (same element type)
 };
 
-// Global work index w of a multi-transform launch: descs[desc] owns it and
-// its first piece is first_piece (an index into that desc's piece table).
+// One work item, everything the gather needs to start on it: the owning
+// transform's piece table and byte scan, the item's byte range [lo, hi) of
+// the piece stream, its first piece, and (multi-transform launches) the
+// index of the owning desc, whose requests apply.
 struct work_rec {
-    uint32_t desc;
+    const piece *pieces;
+    const uint64_t *pend;
+    uint64_t lo, hi;
     uint32_t first_piece;
+    uint32_t n_pieces;
+    uint32_t desc;
+    uint32_t pad;
 };
+static_assert(sizeof(work_rec) == 48, "work_rec layout");
 
 // A launch: one desc passed by value, or a device table of descs with a
 // record for every global work index (multi-transform launches).
@@ -188,8 +196,8 @@ inline int request_features(const request &r) {
 // `cus` = compute units of the device (persistent grids fill them).
 hipError_t launch_gather(const gather_args &a, const launch_cfg &c, int cus, hipStream_t stream);
 
-// work[w] = {k, descs[k].first_piece[w - tile_base]} for w in
-// [descs[k].tile_base, + n_tiles), k < n.
+// work[w] = the record of work index w for w in [descs[k].tile_base, +
+// n_tiles), k < n.
 hipError_t launch_work_records(const work_desc *descs, int32_t n, work_rec *work, hipStream_t stream);
 
 // ------------------------------------------------ host helpers (g2g_host.cpp)

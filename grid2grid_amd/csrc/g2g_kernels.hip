@@ -593,15 +593,17 @@ enum : uint32_t { PK_NONE = 0, PK_VEC = 1, PK_ELEM = 2, PK_DIRECT = 3 };
 // by the whole workgroup. Pieces write disjoint destinations, so their order
 // does not matter.
 template <int THREADS, int UB, bool NT, int FEAT>
-__device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uint32_t p, unsigned char *smem,
+__device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &kr, unsigned char *smem,
                                              seg_entry *segs, seg_entry *tsegs) {
-    const uint64_t lo = (uint64_t)t * a.tile_bytes;
-    const uint64_t hi = min(lo + a.tile_bytes, a.total_bytes);
+    const uint64_t lo = job.lo, hi = job.hi;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t es = a.kr.r[0].esize;  // both requests of a desc share the element type
-    const uint32_t n = a.n_pieces;
+    const uint32_t es = kr.r[0].esize;  // both requests of a desc share the element type
+    const uint32_t n = job.n_pieces;
+    const piece *pieces = job.pieces;
+    const uint64_t *pend = job.pend;
     // p: the work item's first piece; a lane-0 piece past the work item ends
     // the loop below without a separate check
+    uint32_t p = job.first_piece;
     for (;;) {
         if (p >= n) break;
         const uint32_t q = p + lane;
@@ -609,11 +611,11 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uin
         piece pc;
         bool in = q < n;
         if (in) {
-            pc = a.pieces[q];
-            const uint64_t ps = q ? a.pend[q - 1] : 0;
-            const uint64_t pe = a.pend[q];
+            pc = pieces[q];
+            const uint64_t ps = q ? pend[q - 1] : 0;
+            const uint64_t pe = pend[q];
             in = ps < hi;
-            flags = pc.req ? a.kr.r[1].flags : a.kr.r[0].flags;
+            flags = pc.req ? kr.r[1].flags : kr.r[0].flags;
             rows = pc.rows;
             piece_cols(rows, pc.cols, flags, es, ps, pe, lo, hi, c0, c1);
             if (in && c0 < c1) {
@@ -684,10 +686,10 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uin
             while (md) {
                 const uint32_t d = p + (uint32_t)__ffsll((unsigned long long)md) - 1;
                 md &= md - 1;
-                const piece dp = a.pieces[d];
-                const request &rq = dp.req ? a.kr.r[1] : a.kr.r[0];
+                const piece dp = pieces[d];
+                const request &rq = dp.req ? kr.r[1] : kr.r[0];
                 uint32_t d0, d1;
-                piece_cols(dp.rows, dp.cols, rq.flags, es, d ? a.pend[d - 1] : 0, a.pend[d], lo, hi, d0, d1);
+                piece_cols(dp.rows, dp.cols, rq.flags, es, d ? pend[d - 1] : 0, pend[d], lo, hi, d0, d1);
                 process_piece<THREADS, NT, FEAT>(dp, rq, d0, d1, smem);
             }
         }
@@ -698,17 +700,32 @@ __device__ __forceinline__ void process_tile(const work_desc &a, uint32_t t, uin
 
 // Global work index w -> (transform, local work item) -> columns to move.
 template <int THREADS, int UB, bool NT, int FEAT>
-__device__ __forceinline__ void process_work(const gather_args &a, uint32_t w,
+__device__ __forceinline__ void process_work(const gather_args &a, uint32_t w, uint32_t *claim, uint32_t &next,
                                              unsigned char *smem, seg_entry *segs, seg_entry *tsegs) {
+    work_rec job;
     if (!a.descs) {
-        process_tile<THREADS, UB, NT, FEAT>(a.one, w, a.one.first_piece[w], smem, segs, tsegs);
-        return;
+        job.pieces = a.one.pieces;
+        job.pend = a.one.pend;
+        job.lo = (uint64_t)w * a.one.tile_bytes;
+        job.hi = min(job.lo + a.one.tile_bytes, a.one.total_bytes);
+        job.first_piece = a.one.first_piece[w];
+        job.n_pieces = a.one.n_pieces;
+        job.desc = 0;
+    } else {
+        job = a.work[w];  // all the pieces' loads wait on
     }
-    // one record gives both the transform and the first piece, so the
-    // descriptor and the pieces are the only loads that wait on it
-    const work_rec r = a.work[w];
-    const work_desc d = a.descs[r.desc];
-    process_tile<THREADS, UB, NT, FEAT>(d, w - d.tile_base, r.first_piece, smem, segs, tsegs);
+    // Persistent launches claim the next work index here: after the loads
+    // above, so that waiting for them does not wait for the claim too (memory
+    // operations retire in order), and before the work, whose latency then
+    // hides the claim's.
+    asm volatile("" ::: "memory");
+    if (claim && threadIdx.x == 0) next = atomicAdd(claim, 1u);
+    if (!a.descs) {
+        process_tile<THREADS, UB, NT, FEAT>(job, a.one.kr, smem, segs, tsegs);
+    } else {
+        const kreqs kr = a.descs[job.desc].kr;
+        process_tile<THREADS, UB, NT, FEAT>(job, kr, smem, segs, tsegs);
+    }
 }
 
 // Direct mode: one workgroup per work index. Persistent mode: a grid sized to
@@ -718,6 +735,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[kSmemBytes];
     __shared__ seg_entry segs[kMaxSeg + 1];
     __shared__ seg_entry tsegs[FEAT == FEAT_COPY ? 1 : kMaxSeg + 1];
+    uint32_t next = 0;
     if (!PERSIST) {
         uint32_t w = blockIdx.x;
         if (a.xcd_ranges) {
@@ -726,21 +744,18 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
             const uint32_t n = gridDim.x, x = w & 7u, q = n >> 3, r = n & 7u;
             w = x * q + min(x, r) + (w >> 3);
         }
-        process_work<THREADS, UB, NT, FEAT>(a, w, smem, segs, tsegs);
+        process_work<THREADS, UB, NT, FEAT>(a, w, nullptr, next, smem, segs, tsegs);
         return;
     }
     __shared__ uint32_t s_t;
     if (!a.xcd_ranges) {
-        // the next work index is claimed before the current one is moved, so
-        // the counter's latency overlaps the work
+        // the next work index is claimed while the current one is moved
         if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);
         __syncthreads();
         uint32_t t = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
         while (t < a.total_tiles) {
-            uint32_t next = 0;
-            if (threadIdx.x == 0) next = atomicAdd(a.queue, 1u);
-            process_work<THREADS, UB, NT, FEAT>(a, t, smem, segs, tsegs);
+            process_work<THREADS, UB, NT, FEAT>(a, t, a.queue, next, smem, segs, tsegs);
             if (threadIdx.x == 0) s_t = next;
             __syncthreads();
             t = __builtin_amdgcn_readfirstlane(s_t);
@@ -761,9 +776,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
         uint32_t c = __builtin_amdgcn_readfirstlane(s_t);
         __syncthreads();
         while (c < len) {
-            uint32_t next = 0;
-            if (threadIdx.x == 0) next = atomicAdd(a.queue + x, 1u);  // overlaps the work below
-            process_work<THREADS, UB, NT, FEAT>(a, lo + c, smem, segs, tsegs);
+            process_work<THREADS, UB, NT, FEAT>(a, lo + c, a.queue + x, next, smem, segs, tsegs);
             if (threadIdx.x == 0) s_t = next;
             __syncthreads();
             c = __builtin_amdgcn_readfirstlane(s_t);
@@ -824,9 +837,19 @@ hipError_t launch_gather(const gather_args &a_in, const launch_cfg &c, int cus, 
 
 __global__ void k_work_records(const work_desc *descs, work_rec *work) {
     const uint32_t k = blockIdx.x;
-    const uint32_t base = descs[k].tile_base, n = descs[k].n_tiles;
-    const uint32_t *fp = descs[k].first_piece;
-    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) work[base + t] = work_rec{k, fp[t]};
+    const work_desc &d = descs[k];
+    for (uint32_t t = threadIdx.x; t < d.n_tiles; t += blockDim.x) {
+        work_rec r;
+        r.pieces = d.pieces;
+        r.pend = d.pend;
+        r.lo = (uint64_t)t * d.tile_bytes;
+        r.hi = min(r.lo + d.tile_bytes, d.total_bytes);
+        r.first_piece = d.first_piece[t];
+        r.n_pieces = d.n_pieces;
+        r.desc = k;
+        r.pad = 0;
+        work[d.tile_base + t] = r;
+    }
 }
 
 hipError_t launch_work_records(const work_desc *descs, int32_t n, work_rec *work, hipStream_t stream) {

@@ -304,3 +304,28 @@ def test_random_requests_both_forms():
         want = _expected(init, ia, fin, fa, dtype, trans, alpha, beta)
         got = _run(init, ia, fin, fa, dtype, trans, alpha, beta, staged=staged)
         np.testing.assert_array_equal(got, want, err_msg=f"case {k}")
+
+
+def test_beyond_32_bit_element_indices():
+    """65536 x 40000 f32 (2.6e9 elements, 10.5 GB per side): element offsets,
+    piece addresses and the seeded values' indices all pass 2^31 and 2^32.
+    Checked on the device element by element against v(seed, i, j)."""
+    m, n = 65536, 40000
+    init = scalapack_layout(m, n, 4096, 2500, 4, 2, "C", lld="rows")
+    fin = scalapack_layout(m, n, 8192, 5000, 2, 4, "R", lld="rows")
+    dtype = g2g.FLOAT32
+    es = g2g.DTYPE_SIZE[dtype]
+    assert m * n > 2**31 and init.arena_elems * es > 2**33
+    ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device="cuda")
+    tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device="cuda")
+    g2g.fill_layout(init, ti.data_ptr(), dtype, 31)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+    plan.transform()
+    torch.cuda.synchronize()
+    assert g2g.verify_layout(fin, tf.data_ptr(), dtype, 31, "N", n) == 0
+    # the check sees a single flipped element (row 5 of the first block)
+    tf.view(torch.int32)[int(fin.blk_offset[0]) + 5] ^= 1
+    assert g2g.verify_layout(fin, tf.data_ptr(), dtype, 31, "N", n) == 1
+    plan.close()
+    del ti, tf
+    torch.cuda.empty_cache()

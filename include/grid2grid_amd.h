@@ -132,7 +132,10 @@ int g2g_plan_get_info(g2g_plan plan, g2g_plan_info *info);
  * alpha/beta point to one host element of the plan's dtype (NULL, NULL =
  * 1 and 0 = plain copy). Every element of every final block is written;
  * bytes outside the pieces (ld padding, gaps) are never touched.
- * Asynchronous on `stream`. */
+ * Asynchronous on `stream`. A plan holds one work counter and one set of
+ * staging buffers, so launches of the same plan (fused or staged) must not
+ * overlap: issue them on one stream, or order them with events; plans built
+ * for the same layouts can run side by side on different streams. */
 int g2g_transform(g2g_plan plan, const void *alpha, const void *beta, void *stream);
 
 /* The plain form, mirroring the reference step for step: pack every remote
@@ -178,7 +181,8 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
                      const void *const *betas, void *stream, g2g_batch *out);
 
 /* Run every transform of the batch in one persistent kernel that pulls
- * (transform, tile) work items from a device queue. */
+ * (transform, tile) work items from a device queue. Launches of one batch
+ * must not overlap (one work counter per batch). */
 int g2g_batch_execute(g2g_batch batch, void *stream);
 
 /* Same work replayed from a hipGraph captured on first use (the queue reset

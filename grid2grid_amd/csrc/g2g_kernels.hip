@@ -22,6 +22,9 @@ typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 // wait counter.
 #define GLOBAL __attribute__((address_space(1)))
 typedef GLOBAL unsigned char gbyte;
+// Pointers into LDS (address space 3): 32-bit addresses whose constant parts
+// fold into the ds_* instructions' offset fields.
+#define LOCAL __attribute__((address_space(3)))
 
 template <int B> struct bits_t;
 template <> struct bits_t<4> { using type = uint32_t; };
@@ -406,6 +409,17 @@ __device__ __forceinline__ void copy_cols_scaled(const gbyte *__restrict__ s,
 constexpr uint32_t kTileBytes = 16384;  // data bytes of one transpose tile
 constexpr uint32_t kSmemBytes = kTileBytes + 64 * 16;  // plus one padding element per tile row
 
+// Transpose tiles moved per pass of the full-tile path: enough that every
+// thread keeps 128 bytes of loads in flight (a transposing workgroup of 512
+// threads shares a CU with only one other, so one 16 KiB tile per pass left a
+// CU with 32 KiB of loads in flight). The pass's LDS (4 x 17 KiB for 512
+// threads, 2 x 17 KiB for 256) still lets 2 (512) or 3 (256) workgroups share
+// a CU.
+template <int THREADS>
+__host__ __device__ constexpr uint32_t trans_tiles() {
+    return THREADS >= 512 ? 4u : 2u;
+}
+
 // Transposing copy: destination(i, j) = op(source(j, i)) for final rows
 // i < rows and final columns j < ncols. Source element (i, j) lies at
 // s + (i*lds + j)*ES: storage rows run along final columns. A tile is TC final
@@ -416,13 +430,11 @@ constexpr uint32_t kSmemBytes = kTileBytes + 64 * 16;  // plus one padding eleme
 // (dword-aligned addresses); LDS holds the tile in storage order, rows padded
 // by one element against bank conflicts. Vectors that would run past the
 // piece move element by element.
-// Not inlined: a called function keeps the registers of this path out of the
-// queue loops of the gather (fewer registers per wave, more waves per CU).
 template <int THREADS, bool NT, int ES, uint32_t TC, bool SCALE>
-__device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restrict__ s,
+__device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
                                                 gbyte *__restrict__ d, uint64_t lds_b,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
-                                                const request &rq, unsigned char *smem) {
+                                                const request &rq, LOCAL unsigned char *smem) {
     using T = typename bits_t<ES>::type;
     constexpr uint32_t VE = 16 / ES;                     // elements per vector
     constexpr uint32_t TR = kTileBytes / (TC * ES);      // final columns per tile
@@ -433,7 +445,7 @@ __device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restric
     constexpr uint32_t WL = __builtin_ctz(W);
     constexpr uint32_t PER = kTileBytes / 16 / THREADS;  // vectors per thread per phase
     static_assert(TC * LDW * ES <= kSmemBytes, "transpose tile exceeds its LDS");
-    T *tile = reinterpret_cast<T *>(smem);               // tile[c * LDW + r]
+    LOCAL T *tile = (LOCAL T *)smem;               // tile[c * LDW + r]
     const bool conj = (rq.flags & REQ_CONJ) != 0;
     const bool scale = SCALE && (rq.flags & REQ_SCALE) != 0;
     const uint32_t n_ic = (rows + TC - 1) / TC;
@@ -467,7 +479,7 @@ __device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restric
         for (uint32_t k = 0; k < PER; ++k) {
             const uint32_t q = threadIdx.x + k * THREADS;
             const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
-            T *tp = tile + c * LDW + r0;
+            LOCAL T *tp = tile + c * LDW + r0;
 #pragma unroll
             for (uint32_t e = 0; e < VE; ++e)
                 if (e < cnt1[k]) tp[e] = vec_elem<ES>(v1[k], (int)e);
@@ -481,7 +493,7 @@ __device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restric
             const uint32_t j = jc + r, i0 = ic + w * VE;
             if (j < ncols && i0 < rows) {
                 gbyte *p = d + j * ldd_b + (uint64_t)i0 * ES;
-                const T *tp = tile + (w * VE) * LDW + r;
+                const LOCAL T *tp = tile + (w * VE) * LDW + r;
                 if (i0 + VE <= rows) {
                     u32x4 v, old;
                     if (scale) old = ld_a4<false>(p);
@@ -508,18 +520,191 @@ __device__ __attribute__((noinline)) void copy_transposed(const gbyte *__restric
     }
 }
 
-// Tile height for a transposed piece of `rows` final rows.
-template <int THREADS, bool NT, int ES, bool SCALE>
-__device__ __forceinline__ void copy_transposed_rows(const gbyte *__restrict__ s,
-                                                     gbyte *__restrict__ d, uint64_t lds_b,
-                                                     uint64_t ldd_b, uint32_t rows, uint32_t ncols,
-                                                     const request &rq, unsigned char *smem) {
-    if (rows > 32)
-        copy_transposed<THREADS, NT, ES, 64, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
-    else if (rows > 16)
-        copy_transposed<THREADS, NT, ES, 32, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+// A value the caller passes identically to every lane, kept in scalar
+// registers (arguments of a called function arrive in vector registers).
+// readfirstlane returns int: each half goes back through uint32_t, so the low
+// half is zero-extended, never sign-extended into the high one.
+__device__ __forceinline__ uint32_t uniform32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    return ((uint64_t)uniform32((uint32_t)(x >> 32)) << 32) | (uint64_t)uniform32((uint32_t)x);
+}
+
+// The same transposing copy over whole tiles only: n_ic tiles of TC final
+// rows down, n_tiles in all (n_tiles / n_ic column strips of TR final
+// columns), every element of every tile inside the piece, so no vector needs
+// a bounds check. A pass moves G = trans_tiles<THREADS>() consecutive tiles
+// (row tiles first), each through its own LDS area; the global loads of the
+// next pass are issued before the stores of the current one. Addresses are a
+// tile base in scalar registers plus a 32-bit offset per thread and vector,
+// fixed for the whole call: the caller guarantees TC*lds_b and TR*ldd_b
+// below 2^32.
+template <int THREADS, bool NT, int ES, uint32_t TC, bool SCALE>
+__device__ __forceinline__ void copy_transposed_full(const gbyte *__restrict__ s_in,
+                                                     gbyte *__restrict__ d_in, uint64_t lds_in,
+                                                     uint64_t ldd_in, uint32_t n_ic_in, uint32_t n_tiles_in,
+                                                     const request &rq, LOCAL unsigned char *smem) {
+    using T = typename bits_t<ES>::type;
+    constexpr uint32_t G = trans_tiles<THREADS>();
+    constexpr uint32_t VE = 16 / ES;
+    constexpr uint32_t TR = kTileBytes / (TC * ES);
+    constexpr uint32_t LDW = TR + 1;
+    constexpr uint32_t RV = TR / VE;
+    constexpr uint32_t RVL = __builtin_ctz(RV);
+    constexpr uint32_t W = TC * ES / 16;
+    constexpr uint32_t WL = __builtin_ctz(W);
+    constexpr uint32_t PER = kTileBytes / 16 / THREADS;
+    constexpr uint32_t TILE_ELEMS = kSmemBytes / ES;     // LDS area of one tile, in elements
+    static_assert(TC * LDW * ES <= kSmemBytes, "transpose tile exceeds its LDS");
+    const gbyte *s = (const gbyte *)uniform64((uint64_t)s_in);
+    gbyte *d = (gbyte *)uniform64((uint64_t)d_in);
+    const uint64_t lds_b = uniform64(lds_in), ldd_b = uniform64(ldd_in);
+    const uint32_t n_ic = uniform32(n_ic_in), n_tiles = uniform32(n_tiles_in);
+    const bool conj = uniform32(rq.flags & REQ_CONJ) != 0;
+    const bool scale = SCALE && uniform32(rq.flags & REQ_SCALE) != 0;
+    LOCAL T *tile = (LOCAL T *)smem;               // area g: tile[g*TILE_ELEMS + c*LDW + r]
+    // per thread and vector k: source offset inside a tile (storage column,
+    // element), destination offset (final column, row), LDS positions
+    uint32_t so[PER], dof[PER], lw[PER], lr[PER];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t q = threadIdx.x + k * THREADS;
+        const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+        so[k] = c * (uint32_t)lds_b + r0 * ES;
+        lw[k] = c * LDW + r0;
+        const uint32_t r = q >> WL, w = q & (W - 1);
+        dof[k] = r * (uint32_t)ldd_b + w * VE * ES;
+        lr[k] = (w * VE) * LDW + r;
+    }
+    const uint32_t n_pass = (n_tiles + G - 1) / G;
+    u32x4 v[G][PER];
+    auto load_pass = [&](uint32_t pass) {
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g) {
+            const uint32_t t = pass * G + g;
+            if (t < n_tiles) {
+                const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
+                const gbyte *base = s + ic * lds_b + (uint64_t)jc * ES;
+#pragma unroll
+                for (uint32_t k = 0; k < PER; ++k) v[g][k] = ld_a4<NT>(base + so[k]);
+            }
+        }
+    };
+    if (n_pass) load_pass(0);
+    for (uint32_t pass = 0; pass < n_pass; ++pass) {
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g) {
+            if (pass * G + g < n_tiles) {
+#pragma unroll
+                for (uint32_t k = 0; k < PER; ++k) {
+                    LOCAL T *tp = tile + g * TILE_ELEMS + lw[k];
+#pragma unroll
+                    for (uint32_t e = 0; e < VE; ++e) tp[e] = vec_elem<ES>(v[g][k], (int)e);
+                }
+            }
+        }
+        __syncthreads();
+        if (pass + 1 < n_pass) load_pass(pass + 1);
+        // one tile at a time: its LDS and global addresses are computed in
+        // the loop rather than held for every tile of the pass
+#pragma unroll 1
+        for (uint32_t g = 0; g < G; ++g) {
+            const uint32_t t = pass * G + g;
+            if (t < n_tiles) {
+                const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
+                gbyte *base = d + jc * ldd_b + (uint64_t)ic * ES;
+                const LOCAL T *tg = tile + g * TILE_ELEMS;
+#pragma unroll
+                for (uint32_t k = 0; k < PER; ++k) {
+                    gbyte *p = base + dof[k];
+                    const LOCAL T *tp = tg + lr[k];
+                    u32x4 o, old;
+                    if (scale) old = ld_a4<false>(p);
+#pragma unroll
+                    for (uint32_t e = 0; e < VE; ++e) {
+                        T x = tp[e * LDW];
+                        if (conj) x = conj_bits(x);
+                        if (scale) x = scale_bits(x, vec_elem<ES>(old, e), rq);
+                        vec_set<ES>(o, e, x);
+                    }
+                    st_a4<NT>(p, o);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One part of a transposed piece: FULL = its whole tiles (copy_transposed_full,
+// with n_ic = x, n_tiles = y), else a strip through the bounds-checked path
+// (rows = x, ncols = y). Not inlined: a called function keeps the registers of
+// the transpose out of the queue loops of the gather, and the caller keeps
+// its own live values in registers the callee leaves alone, so every callee
+// is kept small: one per part, element size and tile height. The request
+// arrives as scalars: a struct argument would be copied to the caller's
+// private memory.
+template <int THREADS, bool NT, int ES, uint32_t TC, bool SCALE, bool FULL>
+__device__ __attribute__((noinline)) void transpose_part(const gbyte *__restrict__ s, gbyte *__restrict__ d,
+                                                         uint64_t lds_b, uint64_t ldd_b, uint32_t x, uint32_t y,
+                                                         uint32_t flags, uint32_t dtype, uint64_t a0, uint64_t a1,
+                                                         uint64_t b0, uint64_t b1, LOCAL unsigned char *smem) {
+    request rq;
+    rq.dtype = dtype;
+    rq.flags = flags;
+    rq.esize = ES;
+    rq.pad = 0;
+    rq.alpha[0] = a0;
+    rq.alpha[1] = a1;
+    rq.beta[0] = b0;
+    rq.beta[1] = b1;
+    if constexpr (FULL)
+        copy_transposed_full<THREADS, NT, ES, TC, SCALE>(s, d, lds_b, ldd_b, x, y, rq, smem);
     else
-        copy_transposed<THREADS, NT, ES, 16, SCALE>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+        copy_transposed<THREADS, NT, ES, TC, SCALE>(s, d, lds_b, ldd_b, x, y, rq, smem);
+}
+
+// A transposed piece with tiles of TC final rows: whole tiles through the
+// full-tile path, then the bottom strip (the last rows % TC final rows, every
+// column) and the right strip (the last ncols % TR final columns of the other
+// rows) through the bounds-checked one.
+template <int THREADS, bool NT, int ES, uint32_t TC, int FEAT>
+__device__ __forceinline__ void transpose_tc(const gbyte *s, gbyte *d, uint64_t lds_b, uint64_t ldd_b,
+                                             uint32_t rows, uint32_t ncols, const request &rq,
+                                             unsigned char *smem_g) {
+    constexpr bool SC = FEAT == FEAT_ALL;
+    constexpr uint32_t TR = kTileBytes / (TC * ES);
+    LOCAL unsigned char *smem = (LOCAL unsigned char *)smem_g;
+    const uint32_t dt = SC ? rq.dtype : 0u;
+    const uint64_t a0 = SC ? rq.alpha[0] : 0u, a1 = SC ? rq.alpha[1] : 0u;
+    const uint64_t b0 = SC ? rq.beta[0] : 0u, b1 = SC ? rq.beta[1] : 0u;
+    // whole tiles: unless a tile spans 4 GiB or more of either side (32-bit
+    // offsets inside a tile)
+    const bool small_ld = (uint64_t)TC * lds_b < (1ull << 32) && (uint64_t)TR * ldd_b < (1ull << 32);
+    const uint32_t rf = small_ld ? rows / TC * TC : 0u, cf = ncols / TR * TR;
+    if (rf && cf)
+        transpose_part<THREADS, NT, ES, TC, SC, true>(s, d, lds_b, ldd_b, rf / TC, (rf / TC) * (cf / TR), rq.flags,
+                                                     dt, a0, a1, b0, b1, smem);
+    if (rf < rows)
+        transpose_part<THREADS, NT, ES, TC, SC, false>(s + rf * lds_b, d + (uint64_t)rf * ES, lds_b, ldd_b,
+                                                      rows - rf, ncols, rq.flags, dt, a0, a1, b0, b1, smem);
+    if (rf && cf < ncols)
+        transpose_part<THREADS, NT, ES, TC, SC, false>(s + (uint64_t)cf * ES, d + cf * ldd_b, lds_b, ldd_b, rf,
+                                                      ncols - cf, rq.flags, dt, a0, a1, b0, b1, smem);
+}
+
+// A transposed piece of `rows` final rows: tiles of 64 final rows, or 32 / 16
+// for shorter pieces.
+template <int THREADS, bool NT, int ES, int FEAT>
+__device__ __forceinline__ void transpose_call(const gbyte *s, gbyte *d, uint64_t lds_b, uint64_t ldd_b,
+                                               uint32_t rows, uint32_t ncols, const request &rq,
+                                               unsigned char *smem) {
+    if (rows > 32)
+        transpose_tc<THREADS, NT, ES, 64, FEAT>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+    else if (rows > 16)
+        transpose_tc<THREADS, NT, ES, 32, FEAT>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
+    else
+        transpose_tc<THREADS, NT, ES, 16, FEAT>(s, d, lds_b, ldd_b, rows, ncols, rq, smem);
 }
 
 // Transposed or scaled pieces: moved by the whole workgroup right away.
@@ -534,11 +719,11 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
     if (rq.flags & REQ_TRANS) {
         const gbyte *s = (const gbyte *)pc.src + (uint64_t)c0 * es;
         if (es == 4)
-            copy_transposed_rows<THREADS, NT, 4, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            transpose_call<THREADS, NT, 4, FEAT>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else if (es == 8)
-            copy_transposed_rows<THREADS, NT, 8, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            transpose_call<THREADS, NT, 8, FEAT>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         else
-            copy_transposed_rows<THREADS, NT, 16, FEAT == FEAT_ALL>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
+            transpose_call<THREADS, NT, 16, FEAT>(s, d, lds_b, ldd_b, pc.rows, ncols, rq, smem);
         return;
     }
     if constexpr (FEAT != FEAT_ALL) return;  // the host never sends scaling here
@@ -687,7 +872,18 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
                 const uint32_t d = p + (uint32_t)__ffsll((unsigned long long)md) - 1;
                 md &= md - 1;
                 const piece dp = pieces[d];
-                const request &rq = dp.req ? kr.r[1] : kr.r[0];
+                // field by field: selecting one of the two structs by address
+                // would keep both in private memory
+                const bool r1 = dp.req != 0;
+                request rq;
+                rq.dtype = r1 ? kr.r[1].dtype : kr.r[0].dtype;
+                rq.flags = r1 ? kr.r[1].flags : kr.r[0].flags;
+                rq.esize = es;
+                rq.pad = 0;
+                rq.alpha[0] = r1 ? kr.r[1].alpha[0] : kr.r[0].alpha[0];
+                rq.alpha[1] = r1 ? kr.r[1].alpha[1] : kr.r[0].alpha[1];
+                rq.beta[0] = r1 ? kr.r[1].beta[0] : kr.r[0].beta[0];
+                rq.beta[1] = r1 ? kr.r[1].beta[1] : kr.r[0].beta[1];
                 uint32_t d0, d1;
                 piece_cols(dp.rows, dp.cols, rq.flags, es, d ? pend[d - 1] : 0, pend[d], lo, hi, d0, d1);
                 process_piece<THREADS, NT, FEAT>(dp, rq, d0, d1, smem);
@@ -732,9 +928,18 @@ __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w, u
 // the chip pulls work indices from a device counter until all are taken.
 template <bool PERSIST, int THREADS, int UB, bool NT, int FEAT>
 __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[kSmemBytes];
-    __shared__ seg_entry segs[kMaxSeg + 1];
-    __shared__ seg_entry tsegs[FEAT == FEAT_COPY ? 1 : kMaxSeg + 1];
+    // One LDS block, the queues first: their fields then sit at addresses that
+    // fit the ds_* immediate offsets (16 bits), ahead of the transpose tiles
+    // of one pass (unused by the copy-only variant).
+    struct lds_t {
+        seg_entry segs[kMaxSeg + 1];
+        seg_entry tsegs[FEAT == FEAT_COPY ? 1 : kMaxSeg + 1];
+        uint32_t t;  // persistent launches: the next work index
+        __attribute__((aligned(16))) unsigned char smem[FEAT == FEAT_COPY ? 16 : trans_tiles<THREADS>() * kSmemBytes];
+    };
+    __shared__ lds_t lds;
+    seg_entry *segs = lds.segs, *tsegs = lds.tsegs;
+    unsigned char *smem = lds.smem;
     uint32_t next = 0;
     if (!PERSIST) {
         uint32_t w = blockIdx.x;
@@ -747,7 +952,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(gather_args a) {
         process_work<THREADS, UB, NT, FEAT>(a, w, nullptr, next, smem, segs, tsegs);
         return;
     }
-    __shared__ uint32_t s_t;
+    uint32_t &s_t = lds.t;
     if (!a.xcd_ranges) {
         // the next work index is claimed while the current one is moved
         if (threadIdx.x == 0) s_t = atomicAdd(a.queue, 1u);

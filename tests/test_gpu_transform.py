@@ -249,6 +249,34 @@ def test_work_item_sizes(tile_bytes, launch_mode):
     np.testing.assert_array_equal(got_t, wantT)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_transposed_whole_tiles(dtype):
+    """Transposed pieces of 260 (and 264) final rows by 150 (and 146) final
+    columns: whole transpose tiles moved several per pass (the last pass
+    partly filled), then a bottom strip of 4 (8) final rows and a right strip
+    of the remaining final columns; both workgroup sizes (2 and 4 tiles per
+    pass), every launch mode, at work-item sizes that give each workgroup one
+    column group of a piece or whole pieces."""
+    init = scalapack_layout(300, 524, 150, 260, 2, 2, "C")
+    fin = scalapack_layout(524, 300, 524, 300, 1, 1, "C")
+    rng = np.random.default_rng(31)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    for trans in (["T", "C"] if dtype in (g2g.COMPLEX64, g2g.COMPLEX128) else ["T"]):
+        want = _expected(init, ia, fin, fa, dtype, trans)
+        ti = _dev(ia)
+        for threads, tile_bytes, mode in ((256, 65536, 0), (512, 65536, 1), (256, 1 << 20, 2), (512, 1 << 22, 0),
+                                          (512, 1 << 20, 3), (256, 1 << 22, 1)):
+            print(f"whole tiles dtype {dtype} trans {trans} threads {threads} tile_bytes {tile_bytes} mode {mode}",
+                  flush=True)
+            tf = _dev(fa)
+            plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+            plan.set_tuning(tile_bytes, threads, 128, 1, mode)
+            plan.transform()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(tf.cpu().numpy(), want)
+            plan.close()
+
+
 @pytest.mark.parametrize("shape", [(0, 40), (30, 0), (0, 0)])
 def test_empty_matrix_is_a_no_op(shape):
     """An empty dimension plans to no pieces; transforms (fused, staged,

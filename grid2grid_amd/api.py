@@ -27,6 +27,10 @@ def _scalar_bytes(dtype, value):
     return ctypes.create_string_buffer(arr.tobytes(), arr.nbytes)
 
 
+def _tuning_dict(t):
+    return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
+
+
 def _stream(s):
     if s is None:
         return None
@@ -108,18 +112,23 @@ class Plan:
     def tuning(self):
         t = _lib.G2GTuning()
         check(_lib.load().g2g_plan_get_tuning(self.handle, ctypes.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+        return _tuning_dict(t)
 
-    def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, launch_mode=0):
+    def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, launch_mode=0,
+                   column_chunks=0):
         """launch_mode: 0 one workgroup per work item, 1 persistent grid, 2 one
-        workgroup per work item with a contiguous range of work items per XCD."""
-        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode), 0.0)
+        workgroup per work item with a contiguous range of work items per XCD,
+        3 persistent with one counter per XCD. column_chunks: 0 runs the
+        pieces in their own order, w > 0 (untransposed plans) chunks of at
+        most w columns ordered down each column interval."""
+        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode),
+                           int(column_chunks), 0, 0.0)
         check(_lib.load().g2g_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, stream=None, reps=3):
         t = _lib.G2GTuning()
         check(_lib.load().g2g_plan_autotune(self.handle, _stream(stream), reps, ctypes.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+        return _tuning_dict(t)
 
     def send_buffer(self, rank):
         p = ctypes.c_void_p()
@@ -165,7 +174,7 @@ class Batch:
     def autotune(self, stream=None, reps=2):
         t = _lib.G2GTuning()
         check(_lib.load().g2g_batch_autotune(self.handle, _stream(stream), reps, ctypes.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+        return _tuning_dict(t)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -251,7 +251,10 @@ struct g2g_plan_s {
     int64_t m = 0, n = 0;
     std::vector<span> urows, ucols;  // host copy of the interval table
 
-    stage_table fused;               // all pieces, fused gather
+    stage_table fused;               // all pieces in union-cell order (wire order, plain form)
+    stage_table chunk;               // column-chunked pieces (gather stream when use_chunk)
+    bool use_chunk = false;
+    uint32_t chunk_w = 0;            // columns per chunk
     uint64_t class_key = 0;          // shape class of the tuning table
     double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
     dbuf<uint32_t> ranks;            // (src << 16) | dst per piece
@@ -271,6 +274,10 @@ struct g2g_plan_s {
 };
 
 namespace {
+
+// The table the fused gather runs over: the column-chunked stream when the
+// plan has one, else the pieces themselves.
+stage_table &gtab(g2g_plan p) { return p->use_chunk ? p->chunk : p->fused; }
 
 bool persistent_mode(uint32_t n_tiles) {
     const int p = g_launch_mode.load();
@@ -307,6 +314,7 @@ struct tuned_cfg {
     uint64_t tile_bytes;
     launch_cfg cfg;
     double ms;
+    bool chunk = false;  // the gather runs over the column-chunked stream
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
@@ -588,6 +596,95 @@ int g2g_set_tuning(int64_t tile_bytes, int launch_mode) {
     return G2G_OK;
 }
 
+namespace {
+
+// Column-chunk width of the gather's stream for an untransposed plan, or 0 for
+// none. Where a destination column segment does not start and end on 128-byte
+// lines, the line it shares with the piece above or below it in the same
+// final column is partly written by each; in piece order those parts are
+// written by different work items (often on different XCDs), so the line
+// reaches memory partly written, twice. Cut into chunks of w columns, ordered
+// chunk after chunk down the whole column interval, the two parts are written
+// by one work item within one pass of its copy loop, and the line is merged
+// in L2 before it is written back. w keeps a chunk near 8 KiB of columns.
+uint32_t chunk_width(const g2g_plan_s *p, const wlayout &wf, uint32_t es) {
+    if (p->trans != 'N' || p->urows.empty() || p->ucols.empty()) return 0;
+    bool split = false;
+    for (const cell &c : wf.cells)
+        if ((c.ptr % 128) || ((c.ld * es) % 128)) {
+            split = true;
+            break;
+        }
+    if (!split)
+        for (const span &r : p->urows)
+            if (((uint64_t)(r.start - wf.rs[r.fblk]) * es) % 128) {
+                split = true;
+                break;
+            }
+    if (!split) return 0;
+    const uint64_t mean_col = std::max<uint64_t>(1, (uint64_t)p->m * es / p->urows.size());
+    uint32_t w = 1;
+    while (w < 64 && (uint64_t)(2 * w) * mean_col <= 8192) w *= 2;
+    // at most 16M sub-pieces (40 + 8 bytes of tables each)
+    const uint64_t n_ur = p->urows.size();
+    auto subs = [&](uint32_t cw) {
+        uint64_t g = 0;
+        for (const span &c : p->ucols) g += ((uint64_t)(c.end - c.start) + cw - 1) / cw;
+        return g * n_ur;
+    };
+    while (subs(w) > (16ull << 20)) {
+        if (w >= (1u << 30)) return 0;
+        w *= 2;
+    }
+    return w;
+}
+
+// The chunked stream of an untransposed plan (pieces already built), of
+// p->chunk_w columns per chunk; the plan's gather then runs over it.
+int build_chunks(g2g_plan p, hipStream_t s) {
+    const uint32_t w = p->chunk_w;
+    if (!w) return fail(G2G_ERR_INVALID, "internal: plan has no chunk width");
+    if (p->chunk.n || p->fused.n == 0) {
+        p->use_chunk = p->chunk.n != 0;
+        return G2G_OK;
+    }
+    const int64_t n_ur = (int64_t)p->urows.size();
+    std::vector<uint32_t> cuj, cc0;
+    for (size_t uj = 0; uj < p->ucols.size(); ++uj) {
+        const int32_t cols = p->ucols[uj].end - p->ucols[uj].start;
+        for (int32_t c0 = 0; c0 < cols; c0 += (int32_t)w) {
+            cuj.push_back((uint32_t)uj);
+            cc0.push_back((uint32_t)c0);
+        }
+    }
+    const int64_t n_chunks = (int64_t)cuj.size(), n = n_chunks * n_ur;
+    // sub-piece indices, plus the 64 examined at once by a wave, stay within 32 bits
+    if (n > 0xffffff00ll) return fail(G2G_ERR_UNSUPPORTED, "%lld column chunks of %u columns: too many",
+                                      (long long)n, w);
+    dbuf<uint32_t> d_cuj, d_cc0;
+    dbuf<uint64_t> scratch;
+    G2G_TRY(d_cuj.upload(cuj.data(), cuj.size(), s));
+    G2G_TRY(d_cc0.upload(cc0.data(), cc0.size(), s));
+    p->chunk.n = n;
+    G2G_TRY(p->chunk.pieces.alloc((size_t)n));
+    G2G_TRY(p->chunk.pend.alloc((size_t)n));
+    G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
+                                p->chunk.pieces.p, p->chunk.pend.p, s));
+    G2G_TRY(scratch.alloc(scan_scratch_elems(n)));
+    G2G_HIP(inclusive_scan_u64(p->chunk.pend.p, n, scratch.p, s));
+    uint64_t check = 0;
+    if (n) G2G_HIP(hipMemcpyAsync(&check, p->chunk.pend.p + n - 1, sizeof check, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    if (check != p->fused.total)
+        return fail(G2G_ERR_INVALID, "internal: chunks cover %llu bytes, matrix has %llu",
+                    (unsigned long long)check, (unsigned long long)p->fused.total);
+    p->chunk.total = p->fused.total;
+    p->use_chunk = true;
+    return G2G_OK;
+}
+
+}  // namespace
+
 int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dtype, char trans,
                     void *stream, g2g_plan *out) {
     if (!out) return fail(G2G_ERR_INVALID, "out is NULL");
@@ -652,15 +749,21 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     if (check != total)
         return fail(G2G_ERR_INVALID, "internal: pieces cover %llu bytes, matrix has %llu",
                     (unsigned long long)check, (unsigned long long)total);
+    // the column-chunked stream is an option of the tuner: built when the
+    // tuned configuration of this shape class uses it
+    p->chunk_w = chunk_width(p, wf, es);
     // configuration: the tuned one of this shape class, else the defaults
     p->class_key = shape_class(es, trans, total, np);
     tuned_cfg tc;
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
-        G2G_TRY(retile(p->fused, tc.tile_bytes, s));
-        p->fused.cfg = tc.cfg;
+        if (tc.chunk && p->chunk_w) G2G_TRY(build_chunks(p, s));
+        stage_table &gt = gtab(p);
+        G2G_TRY(retile(gt, tc.tile_bytes, s));
+        gt.cfg = tc.cfg;
     } else {
-        G2G_TRY(retile(p->fused, choose_tile_bytes(total), s));
-        p->fused.cfg = default_cfg(p->fused.n_tiles);
+        stage_table &gt = gtab(p);
+        G2G_TRY(retile(gt, choose_tile_bytes(total), s));
+        gt.cfg = default_cfg(gt.n_tiles);
     }
     *out = p;
     p = nullptr;  // released from the guard
@@ -674,11 +777,14 @@ int g2g_plan_destroy(g2g_plan plan) {
 
 int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
     if (!p || !t) return fail(G2G_ERR_INVALID, "NULL argument");
-    t->tile_bytes = (int64_t)p->fused.tile_bytes;
-    t->threads = p->fused.cfg.threads;
-    t->unroll_bytes = p->fused.cfg.unroll_bytes;
-    t->nontemporal = p->fused.cfg.nontemporal;
-    t->launch_mode = launch_mode_of(p->fused.cfg);
+    const stage_table &gt = gtab(p);
+    t->tile_bytes = (int64_t)gt.tile_bytes;
+    t->threads = gt.cfg.threads;
+    t->unroll_bytes = gt.cfg.unroll_bytes;
+    t->nontemporal = gt.cfg.nontemporal;
+    t->launch_mode = launch_mode_of(gt.cfg);
+    t->column_chunks = p->use_chunk ? (int32_t)p->chunk_w : 0;
+    t->reserved = 0;
     t->ms = p->tuned_ms;
     return G2G_OK;
 }
@@ -689,11 +795,29 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) || t->launch_mode < 0 ||
         t->launch_mode > G2G_LAUNCH_PERSISTENT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
-    if ((uint64_t)t->tile_bytes != p->fused.tile_bytes) G2G_TRY(retile(p->fused, (uint64_t)t->tile_bytes, nullptr));
-    p->fused.cfg.threads = t->threads;
-    p->fused.cfg.unroll_bytes = t->unroll_bytes;
-    p->fused.cfg.nontemporal = t->nontemporal != 0;
-    set_launch_mode(p->fused.cfg, t->launch_mode);
+    if (t->column_chunks < 0 || (t->column_chunks > 0 && p->trans != 'N'))
+        return fail(G2G_ERR_INVALID, "column_chunks %d: a width >= 0, and 0 for transposed plans",
+                    (int)t->column_chunks);
+    if (t->column_chunks > 0) {
+        if (p->chunk.n && p->chunk_w != (uint32_t)t->column_chunks) {  // another width: rebuild
+            p->chunk.pieces.release();
+            p->chunk.pend.release();
+            p->chunk.first.release();
+            p->chunk.n = 0;
+            p->chunk.tile_bytes = 0;
+        }
+        p->chunk_w = (uint32_t)t->column_chunks;
+        G2G_TRY(build_chunks(p, nullptr));
+        p->chunk.tile_bytes = 0;  // retiled below
+    } else {
+        p->use_chunk = false;
+    }
+    stage_table &gt = gtab(p);
+    if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
+    gt.cfg.threads = t->threads;
+    gt.cfg.unroll_bytes = t->unroll_bytes;
+    gt.cfg.nontemporal = t->nontemporal != 0;
+    set_launch_mode(gt.cfg, t->launch_mode);
     return G2G_OK;
 }
 
@@ -708,18 +832,23 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     std::memset(&kr, 0, sizeof kr);
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
+    // the sweep runs over the pieces in their own order; the fastest few
+    // configurations are then timed again on it and, where the plan has one,
+    // on the column-chunked stream
+    p->use_chunk = false;
     const uint64_t total = p->fused.total;
     std::vector<uint64_t> sizes;
     for (uint64_t tb = 32768; tb <= (1u << 20); tb *= 2)
         if (sizes.empty() || tb < total * 2) sizes.push_back(tb);
     // time one configuration (the plan is retiled to tb when it differs)
     auto time_cfg = [&](uint64_t tb, const launch_cfg &c, int n, double &ms_out) -> int {
-        if (p->fused.tile_bytes != tb) G2G_TRY(retile(p->fused, tb, s));
-        p->fused.cfg = c;
+        stage_table &gt = gtab(p);
+        if (gt.tile_bytes != tb) G2G_TRY(retile(gt, tb, s));
+        gt.cfg = c;
         gather_args a;
         std::memset(&a, 0, sizeof a);
-        a.one = make_desc(p->fused, kr, 0);
-        a.total_tiles = p->fused.n_tiles;
+        a.one = make_desc(gt, kr, 0);
+        a.total_tiles = gt.n_tiles;
         a.queue = p->queue.p;
         hipError_t he = launch_gather(a, c, device_cus(), s);  // warm-up
         if (he == hipSuccess) he = hipEventRecord(e0, s);
@@ -759,20 +888,25 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
         const size_t top = std::min<size_t>(4, cand.size());
         double best_ms = 1e30;
-        for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
-            double ms = 0;
-            if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
-            if (ms < best_ms) {
-                best_ms = ms;
-                win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms};
+        for (int chunk = 0; chunk < (p->chunk_w ? 2 : 1) && rc == G2G_OK; ++chunk) {
+            if (chunk && (rc = build_chunks(p, s)) != G2G_OK) break;
+            for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
+                double ms = 0;
+                if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, chunk != 0};
+                }
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != G2G_OK) return rc;
-    G2G_TRY(retile(p->fused, win.tile_bytes, s));
-    p->fused.cfg = win.cfg;
+    p->use_chunk = win.chunk;
+    stage_table &gt = gtab(p);
+    G2G_TRY(retile(gt, win.tile_bytes, s));
+    gt.cfg = win.cfg;
     p->tuned_ms = win.ms;
     store_tuned(p->class_key, win);
     if (best) G2G_TRY(g2g_plan_get_tuning(p, best));
@@ -787,8 +921,8 @@ int g2g_plan_get_info(g2g_plan p, g2g_plan_info *info) {
     info->n_pieces = p->fused.n;
     info->bytes = (int64_t)p->fused.total;
     info->remote_bytes = p->wire_ready ? p->remote_elems * (int64_t)p->es : -1;
-    info->n_tiles = p->fused.n_tiles;
-    info->tile_bytes = (int64_t)p->fused.tile_bytes;
+    info->n_tiles = gtab(p).n_tiles;
+    info->tile_bytes = (int64_t)gtab(p).tile_bytes;
     info->n_ranks = p->P;
     info->dtype = p->dtype;
     info->trans = p->trans;
@@ -802,7 +936,7 @@ int g2g_transform(g2g_plan p, const void *alpha, const void *beta, void *stream)
     std::memset(&kr, 0, sizeof kr);
     kr.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, true);
     kr.r[1] = kr.r[0];
-    return run_table(p, p->fused, kr, (hipStream_t)stream);
+    return run_table(p, gtab(p), kr, (hipStream_t)stream);
 }
 
 int g2g_transform_staged(g2g_plan p, const void *alpha, const void *beta, void *stream) {
@@ -931,14 +1065,14 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
     uint64_t tiles = 0;
     for (size_t k = 0; k < b->plans.size(); ++k) {
         off[k] = tiles;
-        tiles += (b->plans[k]->fused.total + tile_bytes - 1) / tile_bytes;
+        tiles += (gtab(b->plans[k]).total + tile_bytes - 1) / tile_bytes;
     }
     if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
     b->total_tiles = (uint32_t)tiles;
     G2G_TRY(b->first.alloc(tiles));
     G2G_TRY(b->work.alloc(tiles));
     for (size_t k = 0; k < b->plans.size(); ++k) {
-        const stage_table &t = b->plans[k]->fused;
+        const stage_table &t = gtab(b->plans[k]);
         work_desc &d = b->host_descs[k];
         d.first_piece = b->first.p + off[k];
         d.tile_bytes = tile_bytes;
@@ -969,9 +1103,9 @@ int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas, c
         std::memset(&kr, 0, sizeof kr);
         kr.r[0] = make_request(plans[k]->dtype, plans[k]->trans, a, be, true, true);
         kr.r[1] = kr.r[0];
-        out.push_back(make_desc(plans[k]->fused, kr, (uint32_t)tiles));
-        tiles += plans[k]->fused.n_tiles;
-        bytes += plans[k]->fused.total;
+        out.push_back(make_desc(gtab(plans[k]), kr, (uint32_t)tiles));
+        tiles += gtab(plans[k]).n_tiles;
+        bytes += gtab(plans[k]).total;
         if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
     }
     total_tiles = (uint32_t)tiles;
@@ -1133,6 +1267,8 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->unroll_bytes = b->cfg.unroll_bytes;
         best->nontemporal = b->cfg.nontemporal;
         best->launch_mode = launch_mode_of(b->cfg);
+        best->column_chunks = 0;
+        best->reserved = 0;
         best->ms = b->tuned_ms;
     }
     return G2G_OK;

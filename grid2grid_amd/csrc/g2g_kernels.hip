@@ -1116,6 +1116,36 @@ hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
     return hipGetLastError();
 }
 
+// Sub-piece k = g * n_ur + ui: columns [c0, c0 + w) of untransposed piece
+// (ui, uj), (uj, c0) = chunk g. The source and destination both advance by
+// whole columns (column-major on both sides).
+__global__ void k_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,
+                               const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t es,
+                               piece *out, uint64_t *bytes) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_chunks * n_ur) return;
+    const int64_t g = k / n_ur;
+    const int32_t ui = (int32_t)(k - g * n_ur);
+    const uint32_t uj = chunk_uj[g], c0 = chunk_c0[g];
+    piece pc = pieces[(int64_t)uj * n_ur + ui];
+    const uint32_t nc = min(w, pc.cols - c0);
+    pc.src += (uint64_t)c0 * pc.lds * es;
+    pc.dst += (uint64_t)c0 * pc.ldd * es;
+    pc.cols = nc;
+    out[k] = pc;
+    bytes[k] = (uint64_t)pc.rows * nc * es;
+}
+
+hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,
+                               const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t esize,
+                               piece *out, uint64_t *bytes, hipStream_t stream) {
+    const int64_t n = n_chunks * n_ur;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_pieces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pieces, n_ur,
+                       chunk_uj, chunk_c0, n_chunks, w, esize, out, bytes);
+    return hipGetLastError();
+}
+
 __global__ void k_piece_bytes(const piece *pieces, int64_t n, kreqs kr, uint64_t *bytes) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;

@@ -277,6 +277,73 @@ def test_transposed_whole_tiles(dtype):
             plan.close()
 
 
+@pytest.mark.parametrize("name", ["bc_small", "bc_grid_change", "arbitrary", "tall_thin", "many_ranks"])
+def test_column_chunked_stream(name):
+    """The gather over column chunks (pieces cut into chunks of w columns,
+    ordered down each column interval) writes exactly what the pieces in
+    their own order write, for odd widths and every launch mode; the width is
+    reported back; transposed plans refuse it."""
+    init, fin = _layout_pairs()[name]
+    dtype = g2g.FLOAT32
+    rng = np.random.default_rng(17)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    want = _expected(init, ia, fin, fa, dtype)
+    ti = _dev(ia)
+    for w, mode, tile in ((1, 0, 4096), (3, 1, 65536), (16, 2, 4096), (64, 3, 1 << 20), (3, 0, 1 << 20)):
+        print(f"chunks {name} w {w} mode {mode} tile {tile}", flush=True)
+        tf = _dev(fa)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w)
+        assert plan.tuning["column_chunks"] == w
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want)
+        # back to the pieces in their own order
+        tf.copy_(_dev(fa))
+        plan.set_tuning(tile, 256, 64, 1, mode, column_chunks=0)
+        assert plan.tuning["column_chunks"] == 0
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want)
+        plan.close()
+    initT, finT = _transposed_pairs()["bcT"]
+    planT = g2g.Plan(initT, ti.data_ptr(), finT, ti.data_ptr(), dtype, "T")
+    with pytest.raises(g2g.G2GError):
+        planT.set_tuning(65536, 256, 64, 0, 0, column_chunks=4)
+    planT.close()
+
+
+def test_column_chunked_batch_and_autotune():
+    """A batch of chunked and unchunked plans gives every request's own
+    result; the autotuner's choice (chunked or not) too."""
+    dtype = g2g.FLOAT64
+    rng = np.random.default_rng(23)
+    cases = []
+    for k, name in enumerate(["bc_small", "bc_grid_change", "arbitrary", "tall_thin"]):
+        init, fin = _layout_pairs()[name]
+        ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+        ti, tf = _dev(ia), _dev(fa)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+        if k % 2 == 0:
+            plan.set_tuning(65536, 256, 64, 0, 1, column_chunks=2 + k)
+        cases.append((plan, ti, tf, fa, _expected(init, ia, fin, fa, dtype)))
+    batch = g2g.Batch([c[0] for c in cases])
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, (plan, ti, tf, fa, want) in enumerate(cases):
+        np.testing.assert_array_equal(tf.cpu().numpy(), want, err_msg=f"batched request {k}")
+    batch.close()
+    for k, (plan, ti, tf, fa, want) in enumerate(cases):
+        tf.copy_(_dev(fa))
+        best = plan.autotune(reps=1)
+        print(f"autotuned request {k}: {best}", flush=True)
+        tf.copy_(_dev(fa))
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want, err_msg=f"autotuned request {k}")
+        plan.close()
+
+
 @pytest.mark.parametrize("shape", [(0, 40), (30, 0), (0, 0)])
 def test_empty_matrix_is_a_no_op(shape):
     """An empty dimension plans to no pieces; transforms (fused, staged,

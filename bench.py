@@ -34,8 +34,8 @@ def parse():
                     help="cfg1 | cfg2 | cfg3 | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
-                    help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode "
-                         "(skips the autotuner)")
+                    help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode"
+                         "[:column_chunks] (skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -362,8 +362,8 @@ def main():
         f"planned in {time.time() - t0:.2f}s")
     tuning = plan.tuning
     if args.tuning:
-        tb, th, ub, nt, pe = (int(x) for x in args.tuning.split(":"))
-        plan.set_tuning(tb, th, ub, nt, pe)
+        f = [int(x) for x in args.tuning.split(":")]
+        plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0)
         tuning = plan.tuning
         info = plan.info
     elif args.mode == "fused" and not args.no_autotune:
@@ -404,7 +404,8 @@ def main():
             "pieces": int(info.n_pieces),
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
-            "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode")},
+            "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode",
+                                                      "column_chunks")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

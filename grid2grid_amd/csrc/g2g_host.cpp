@@ -314,7 +314,7 @@ struct tuned_cfg {
     uint64_t tile_bytes;
     launch_cfg cfg;
     double ms;
-    bool chunk = false;  // the gather runs over the column-chunked stream
+    uint32_t chunk_w = 0;  // > 0: the gather runs over column chunks of this width
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
@@ -598,56 +598,38 @@ int g2g_set_tuning(int64_t tile_bytes, int launch_mode) {
 
 namespace {
 
-// Column-chunk width of the gather's stream for an untransposed plan, or 0 for
-// none. Where a destination column segment does not start and end on 128-byte
-// lines, the line it shares with the piece above or below it in the same
-// final column is partly written by each; in piece order those parts are
-// written by different work items (often on different XCDs), so the line
-// reaches memory partly written, twice. Cut into chunks of w columns, ordered
-// chunk after chunk down the whole column interval, the two parts are written
-// by one work item within one pass of its copy loop, and the line is merged
-// in L2 before it is written back. w keeps a chunk near 8 KiB of columns.
-uint32_t chunk_width(const g2g_plan_s *p, const wlayout &wf, uint32_t es) {
-    if (p->trans != 'N' || p->urows.empty() || p->ucols.empty()) return 0;
-    bool split = false;
-    for (const cell &c : wf.cells)
-        if ((c.ptr % 128) || ((c.ld * es) % 128)) {
-            split = true;
-            break;
-        }
-    if (!split)
-        for (const span &r : p->urows)
-            if (((uint64_t)(r.start - wf.rs[r.fblk]) * es) % 128) {
-                split = true;
-                break;
-            }
-    if (!split) return 0;
-    const uint64_t mean_col = std::max<uint64_t>(1, (uint64_t)p->m * es / p->urows.size());
-    uint32_t w = 1;
-    while (w < 64 && (uint64_t)(2 * w) * mean_col <= 8192) w *= 2;
-    // at most 16M sub-pieces (40 + 8 bytes of tables each)
-    const uint64_t n_ur = p->urows.size();
-    auto subs = [&](uint32_t cw) {
-        uint64_t g = 0;
-        for (const span &c : p->ucols) g += ((uint64_t)(c.end - c.start) + cw - 1) / cw;
-        return g * n_ur;
-    };
-    while (subs(w) > (16ull << 20)) {
-        if (w >= (1u << 30)) return 0;
-        w *= 2;
-    }
-    return w;
+// Sub-pieces of the column-chunked stream of width w (n_ur per chunk).
+uint64_t chunk_count(const g2g_plan_s *p, uint32_t w) {
+    uint64_t g = 0;
+    for (const span &c : p->ucols) g += ((uint64_t)(c.end - c.start) + w - 1) / w;
+    return g * p->urows.size();
 }
 
-// The chunked stream of an untransposed plan (pieces already built), of
-// p->chunk_w columns per chunk; the plan's gather then runs over it.
-int build_chunks(g2g_plan p, hipStream_t s) {
-    const uint32_t w = p->chunk_w;
-    if (!w) return fail(G2G_ERR_INVALID, "internal: plan has no chunk width");
-    if (p->chunk.n || p->fused.n == 0) {
-        p->use_chunk = p->chunk.n != 0;
+// The gather's stream of an untransposed plan: w = 0, the pieces in their
+// own order; w > 0, the pieces cut into chunks of at most w columns, ordered
+// chunk after chunk down each column interval (built on first use, rebuilt
+// for another width). Vertically adjacent pieces then move the same columns
+// at about the same time: their destination segments, contiguous within a
+// final block column, are written together (longer DRAM runs, and the cache
+// line two pieces share when a segment does not end on a 128-byte line is
+// merged in L2 instead of reaching memory partly written, twice).
+int select_chunks(g2g_plan p, uint32_t w, hipStream_t s) {
+    if (w == 0 || p->fused.n == 0) {
+        p->use_chunk = false;
         return G2G_OK;
     }
+    if (p->trans != 'N') return fail(G2G_ERR_INVALID, "column chunks apply to untransposed plans");
+    if (p->chunk.n && p->chunk_w == w) {
+        p->use_chunk = true;
+        return G2G_OK;
+    }
+    p->use_chunk = false;
+    p->chunk.pieces.release();
+    p->chunk.pend.release();
+    p->chunk.first.release();
+    p->chunk.n = 0;
+    p->chunk.tile_bytes = 0;
+    p->chunk_w = w;
     const int64_t n_ur = (int64_t)p->urows.size();
     std::vector<uint32_t> cuj, cc0;
     for (size_t uj = 0; uj < p->ucols.size(); ++uj) {
@@ -665,7 +647,6 @@ int build_chunks(g2g_plan p, hipStream_t s) {
     dbuf<uint64_t> scratch;
     G2G_TRY(d_cuj.upload(cuj.data(), cuj.size(), s));
     G2G_TRY(d_cc0.upload(cc0.data(), cc0.size(), s));
-    p->chunk.n = n;
     G2G_TRY(p->chunk.pieces.alloc((size_t)n));
     G2G_TRY(p->chunk.pend.alloc((size_t)n));
     G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
@@ -678,7 +659,10 @@ int build_chunks(g2g_plan p, hipStream_t s) {
     if (check != p->fused.total)
         return fail(G2G_ERR_INVALID, "internal: chunks cover %llu bytes, matrix has %llu",
                     (unsigned long long)check, (unsigned long long)p->fused.total);
+    p->chunk.n = n;  // built
     p->chunk.total = p->fused.total;
+    p->chunk.tile_bytes = 0;  // not tiled yet
+    p->chunk.cfg = p->fused.cfg;
     p->use_chunk = true;
     return G2G_OK;
 }
@@ -749,14 +733,12 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     if (check != total)
         return fail(G2G_ERR_INVALID, "internal: pieces cover %llu bytes, matrix has %llu",
                     (unsigned long long)check, (unsigned long long)total);
-    // the column-chunked stream is an option of the tuner: built when the
-    // tuned configuration of this shape class uses it
-    p->chunk_w = chunk_width(p, wf, es);
-    // configuration: the tuned one of this shape class, else the defaults
+    // configuration: the tuned one of this shape class (stream included),
+    // else the defaults
     p->class_key = shape_class(es, trans, total, np);
     tuned_cfg tc;
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
-        if (tc.chunk && p->chunk_w) G2G_TRY(build_chunks(p, s));
+        if (tc.chunk_w && trans == 'N') G2G_TRY(select_chunks(p, tc.chunk_w, s));
         stage_table &gt = gtab(p);
         G2G_TRY(retile(gt, tc.tile_bytes, s));
         gt.cfg = tc.cfg;
@@ -798,20 +780,7 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
     if (t->column_chunks < 0 || (t->column_chunks > 0 && p->trans != 'N'))
         return fail(G2G_ERR_INVALID, "column_chunks %d: a width >= 0, and 0 for transposed plans",
                     (int)t->column_chunks);
-    if (t->column_chunks > 0) {
-        if (p->chunk.n && p->chunk_w != (uint32_t)t->column_chunks) {  // another width: rebuild
-            p->chunk.pieces.release();
-            p->chunk.pend.release();
-            p->chunk.first.release();
-            p->chunk.n = 0;
-            p->chunk.tile_bytes = 0;
-        }
-        p->chunk_w = (uint32_t)t->column_chunks;
-        G2G_TRY(build_chunks(p, nullptr));
-        p->chunk.tile_bytes = 0;  // retiled below
-    } else {
-        p->use_chunk = false;
-    }
+    G2G_TRY(select_chunks(p, (uint32_t)t->column_chunks, nullptr));
     stage_table &gt = gtab(p);
     if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
     gt.cfg.threads = t->threads;
@@ -833,8 +802,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
     // the sweep runs over the pieces in their own order; the fastest few
-    // configurations are then timed again on it and, where the plan has one,
-    // on the column-chunked stream
+    // configurations are then timed again on it and, for untransposed plans,
+    // on column-chunked streams of several widths
     p->use_chunk = false;
     const uint64_t total = p->fused.total;
     std::vector<uint64_t> sizes;
@@ -888,22 +857,27 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
         const size_t top = std::min<size_t>(4, cand.size());
         double best_ms = 1e30;
-        for (int chunk = 0; chunk < (p->chunk_w ? 2 : 1) && rc == G2G_OK; ++chunk) {
-            if (chunk && (rc = build_chunks(p, s)) != G2G_OK) break;
+        std::vector<uint32_t> widths{0};
+        if (p->trans == 'N')
+            for (uint32_t w : {8u, 16u, 32u, 64u})
+                if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
+        for (uint32_t w : widths) {
+            if ((rc = select_chunks(p, w, s)) != G2G_OK) break;
             for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
                 double ms = 0;
                 if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, chunk != 0};
+                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, w};
                 }
             }
+            if (rc != G2G_OK) break;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != G2G_OK) return rc;
-    p->use_chunk = win.chunk;
+    G2G_TRY(select_chunks(p, win.chunk_w, s));
     stage_table &gt = gtab(p);
     G2G_TRY(retile(gt, win.tile_bytes, s));
     gt.cfg = win.cfg;

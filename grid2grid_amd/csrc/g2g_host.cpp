@@ -605,8 +605,8 @@ uint64_t chunk_count(const g2g_plan_s *p, uint32_t w) {
     return g * p->urows.size();
 }
 
-// The gather's stream of an untransposed plan: w = 0, the pieces in their
-// own order; w > 0, the pieces cut into chunks of at most w columns, ordered
+// The gather's stream of a plan: w = 0, the pieces in their own order;
+// w > 0, the pieces cut into chunks of at most w final columns, ordered
 // chunk after chunk down each column interval (built on first use, rebuilt
 // for another width). Vertically adjacent pieces then move the same columns
 // at about the same time: their destination segments, contiguous within a
@@ -618,7 +618,6 @@ int select_chunks(g2g_plan p, uint32_t w, hipStream_t s) {
         p->use_chunk = false;
         return G2G_OK;
     }
-    if (p->trans != 'N') return fail(G2G_ERR_INVALID, "column chunks apply to untransposed plans");
     if (p->chunk.n && p->chunk_w == w) {
         p->use_chunk = true;
         return G2G_OK;
@@ -650,7 +649,7 @@ int select_chunks(g2g_plan p, uint32_t w, hipStream_t s) {
     G2G_TRY(p->chunk.pieces.alloc((size_t)n));
     G2G_TRY(p->chunk.pend.alloc((size_t)n));
     G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
-                                p->chunk.pieces.p, p->chunk.pend.p, s));
+                                p->trans != 'N', p->chunk.pieces.p, p->chunk.pend.p, s));
     G2G_TRY(scratch.alloc(scan_scratch_elems(n)));
     G2G_HIP(inclusive_scan_u64(p->chunk.pend.p, n, scratch.p, s));
     uint64_t check = 0;
@@ -738,7 +737,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     p->class_key = shape_class(es, trans, total, np);
     tuned_cfg tc;
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
-        if (tc.chunk_w && trans == 'N') G2G_TRY(select_chunks(p, tc.chunk_w, s));
+        if (tc.chunk_w) G2G_TRY(select_chunks(p, tc.chunk_w, s));
         stage_table &gt = gtab(p);
         G2G_TRY(retile(gt, tc.tile_bytes, s));
         gt.cfg = tc.cfg;
@@ -777,9 +776,7 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) || t->launch_mode < 0 ||
         t->launch_mode > G2G_LAUNCH_PERSISTENT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
-    if (t->column_chunks < 0 || (t->column_chunks > 0 && p->trans != 'N'))
-        return fail(G2G_ERR_INVALID, "column_chunks %d: a width >= 0, and 0 for transposed plans",
-                    (int)t->column_chunks);
+    if (t->column_chunks < 0) return fail(G2G_ERR_INVALID, "column_chunks %d < 0", (int)t->column_chunks);
     G2G_TRY(select_chunks(p, (uint32_t)t->column_chunks, nullptr));
     stage_table &gt = gtab(p);
     if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
@@ -802,8 +799,9 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
     // the sweep runs over the pieces in their own order; the fastest few
-    // configurations are then timed again on it and, for untransposed plans,
-    // on column-chunked streams of several widths
+    // configurations are then timed again on it and on column-chunked streams
+    // of several widths (transposed plans: whole groups of 256 bytes of final
+    // columns, the width of a transpose tile of 64 final rows)
     p->use_chunk = false;
     const uint64_t total = p->fused.total;
     std::vector<uint64_t> sizes;
@@ -858,9 +856,10 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         const size_t top = std::min<size_t>(4, cand.size());
         double best_ms = 1e30;
         std::vector<uint32_t> widths{0};
-        if (p->trans == 'N')
-            for (uint32_t w : {8u, 16u, 32u, 64u})
-                if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
+        const uint32_t q = 256 / p->es;
+        for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{8, 16, 32, 64}
+                                           : std::vector<uint32_t>{q, 2 * q, 4 * q}))
+            if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
         for (uint32_t w : widths) {
             if ((rc = select_chunks(p, w, s)) != G2G_OK) break;
             for (size_t k = 0; k < top && rc == G2G_OK; ++k) {

@@ -96,13 +96,13 @@ hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
                               piece *pieces, uint64_t *bytes, uint32_t *ranks,
                               hipStream_t stream);
 
-// The untransposed pieces cut into column chunks of at most w columns and
-// ordered chunk after chunk down each column interval: sub-piece
-// k = g * n_ur + ui holds columns [chunk_c0[g], + w) of piece
-// (ui, chunk_uj[g]) (pieces indexed uj * n_ur + ui); bytes[k] its size.
+// The pieces cut into chunks of at most w final columns and ordered chunk
+// after chunk down each column interval: sub-piece k = g * n_ur + ui holds
+// final columns [chunk_c0[g], + w) of piece (ui, chunk_uj[g]) (pieces indexed
+// uj * n_ur + ui; trans: the pieces' sources are transposed); bytes[k] its size.
 hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,
                                const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t esize,
-                               piece *out, uint64_t *bytes, hipStream_t stream);
+                               int trans, piece *out, uint64_t *bytes, hipStream_t stream);
 
 // bytes[p] = rows*cols*esize of the piece's request.
 hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const kreqs &kr, uint64_t *bytes,

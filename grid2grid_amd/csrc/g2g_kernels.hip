@@ -1116,12 +1116,13 @@ hipError_t launch_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
     return hipGetLastError();
 }
 
-// Sub-piece k = g * n_ur + ui: columns [c0, c0 + w) of untransposed piece
-// (ui, uj), (uj, c0) = chunk g. The source and destination both advance by
-// whole columns (column-major on both sides).
+// Sub-piece k = g * n_ur + ui: final columns [c0, c0 + w) of piece (ui, uj),
+// (uj, c0) = chunk g. The destination advances by whole columns; so does the
+// source of an untransposed piece, while a transposed piece's source storage
+// rows run along final columns (element (i, j) at src + (i*lds + j)*es).
 __global__ void k_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,
                                const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t es,
-                               piece *out, uint64_t *bytes) {
+                               int trans, piece *out, uint64_t *bytes) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_chunks * n_ur) return;
     const int64_t g = k / n_ur;
@@ -1129,7 +1130,7 @@ __global__ void k_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
     const uint32_t uj = chunk_uj[g], c0 = chunk_c0[g];
     piece pc = pieces[(int64_t)uj * n_ur + ui];
     const uint32_t nc = min(w, pc.cols - c0);
-    pc.src += (uint64_t)c0 * pc.lds * es;
+    pc.src += trans ? (uint64_t)c0 * es : (uint64_t)c0 * pc.lds * es;
     pc.dst += (uint64_t)c0 * pc.ldd * es;
     pc.cols = nc;
     out[k] = pc;
@@ -1138,11 +1139,11 @@ __global__ void k_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
 
 hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,
                                const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t esize,
-                               piece *out, uint64_t *bytes, hipStream_t stream) {
+                               int trans, piece *out, uint64_t *bytes, hipStream_t stream) {
     const int64_t n = n_chunks * n_ur;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_chunk_pieces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pieces, n_ur,
-                       chunk_uj, chunk_c0, n_chunks, w, esize, out, bytes);
+                       chunk_uj, chunk_c0, n_chunks, w, esize, trans, out, bytes);
     return hipGetLastError();
 }
 

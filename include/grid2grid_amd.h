@@ -275,11 +275,13 @@ typedef struct g2g_tuning {
     int32_t unroll_bytes;  /* bytes each thread keeps in flight: 64 or 128 */
     int32_t nontemporal;   /* 1: non-temporal loads/stores */
     int32_t launch_mode;   /* G2G_LAUNCH_* */
-    /* 0: the stream is the pieces in their own order; w > 0 (untransposed
-     * plans only): the pieces cut into chunks of at most w columns, chunk
-     * after chunk down each column interval, so that vertically adjacent
-     * pieces write the cache lines they share in one work item. Batches
-     * run each plan's stream as the plan is tuned (their own tuning reports 0). */
+    /* 0: the stream is the pieces in their own order; w > 0: the pieces cut
+     * into chunks of at most w final columns, chunk after chunk down each
+     * column interval, so that vertically adjacent pieces write their
+     * destination column segments (contiguous within a final block, sharing
+     * a cache line where a segment does not end on one) at the same time.
+     * Batches run each plan's stream as the plan is tuned (their own tuning
+     * reports 0). */
     int32_t column_chunks;
     int32_t reserved;      /* 0 */
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
@@ -291,9 +293,7 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
 /* Autotuner: time every compiled variant (workgroup size, bytes in flight,
  * non-temporal hint, launch mode) at several work-item sizes on this plan
  * (`reps` timed runs each, on `stream`, hipEvents), time the fastest few
- * again, also on a column-chunked stream when destination column segments
- * of the plan share cache lines (an untransposed plan whose destination
- * blocks, leading dimensions or piece boundaries are not 128-byte aligned),
+ * again, on the pieces and on column-chunked streams of several widths,
  * keep the fastest for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform

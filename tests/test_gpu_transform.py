@@ -306,11 +306,37 @@ def test_column_chunked_stream(name):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         plan.close()
-    initT, finT = _transposed_pairs()["bcT"]
-    planT = g2g.Plan(initT, ti.data_ptr(), finT, ti.data_ptr(), dtype, "T")
     with pytest.raises(g2g.G2GError):
-        planT.set_tuning(65536, 256, 64, 0, 0, column_chunks=4)
-    planT.close()
+        plan = g2g.Plan(init, ti.data_ptr(), fin, ti.data_ptr(), dtype)
+        try:
+            plan.set_tuning(65536, 256, 64, 0, 0, column_chunks=-1)
+        finally:
+            plan.close()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", ["bcT", "arbT", "bcT_mixed_paths", "bcT_short_tiles", "bcT_mid_tiles"])
+def test_column_chunked_transposed(name, dtype):
+    """Transposed pieces cut into chunks of final columns (source offsets
+    along the storage rows): whole LDS tiles, strips and the element queue
+    give the oracle's bytes for chunk widths that split tiles and groups."""
+    init, fin = _transposed_pairs()[name]
+    rng = np.random.default_rng(41)
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    trans = "C" if dtype in (g2g.COMPLEX64, g2g.COMPLEX128) else "T"
+    want = _expected(init, ia, fin, fa, dtype, trans)
+    ti = _dev(ia)
+    q = 256 // g2g.DTYPE_SIZE[dtype]
+    for w, mode, tile in ((q, 0, 65536), (3, 1, 4096), (2 * q, 2, 1 << 20), (40, 3, 65536)):
+        print(f"transposed chunks {name} dtype {dtype} w {w} mode {mode}", flush=True)
+        tf = _dev(fa)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+        plan.set_tuning(tile, 512, 128, 1, mode, column_chunks=w)
+        assert plan.tuning["column_chunks"] == w
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want)
+        plan.close()
 
 
 def test_column_chunked_batch_and_autotune():

@@ -154,8 +154,10 @@ struct work_desc {
 
 // One work item, everything the gather needs to start on it: the owning
 // transform's piece table and byte scan, the item's byte range [lo, hi) of
-// the piece stream, its first piece, and (multi-transform launches) the
-// index of the owning desc, whose requests apply.
+// the piece stream, its first piece and the last piece that can hold one of
+// its columns (the next work item's first piece, or the table's last), and
+// (multi-transform launches) the index of the owning desc, whose requests
+// apply.
 struct work_rec {
     const piece *pieces;
     const uint64_t *pend;
@@ -163,7 +165,7 @@ struct work_rec {
     uint32_t first_piece;
     uint32_t n_pieces;
     uint32_t desc;
-    uint32_t pad;
+    uint32_t last_piece;
 };
 static_assert(sizeof(work_rec) == 48, "work_rec layout");
 

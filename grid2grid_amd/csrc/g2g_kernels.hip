@@ -783,7 +783,8 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
     const uint64_t lo = job.lo, hi = job.hi;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t es = kr.r[0].esize;  // both requests of a desc share the element type
-    const uint32_t n = job.n_pieces;
+    // pieces past the work item's last possible one are not even loaded
+    const uint32_t n = job.last_piece + 1;
     const piece *pieces = job.pieces;
     const uint64_t *pend = job.pend;
     // p: the work item's first piece; a lane-0 piece past the work item ends
@@ -907,6 +908,7 @@ __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w, u
         job.first_piece = a.one.first_piece[w];
         job.n_pieces = a.one.n_pieces;
         job.desc = 0;
+        job.last_piece = w + 1 < a.one.n_tiles ? a.one.first_piece[w + 1] : a.one.n_pieces - 1;
     } else {
         job = a.work[w];  // all the pieces' loads wait on
     }
@@ -1052,7 +1054,7 @@ __global__ void k_work_records(const work_desc *descs, work_rec *work) {
         r.first_piece = d.first_piece[t];
         r.n_pieces = d.n_pieces;
         r.desc = k;
-        r.pad = 0;
+        r.last_piece = t + 1 < d.n_tiles ? d.first_piece[t + 1] : d.n_pieces - 1;
         work[d.tile_base + t] = r;
     }
 }

@@ -421,6 +421,19 @@ def main():
         },
     }
 
+    # streaming reference rates of this device on the same buffers (16-byte
+    # vectors, 16 workgroups per CU): context for the roofline fraction
+    nst = (min(ti.numel(), tf.numel()) // 16) * 16
+    st_ms = {k: g2g.stream_probe(k, ti.data_ptr(), tf.data_ptr(), nst, reps=10, stream=stream) for k in (0, 1, 2)}
+    result["device_stream"] = {
+        "bytes": int(nst),
+        "copy_GB/s": round(2 * nst / (st_ms[0] * 1e-3) / 1e9, 1),
+        "read_GB/s": round(nst / (st_ms[1] * 1e-3) / 1e9, 1),
+        "write_GB/s": round(nst / (st_ms[2] * 1e-3) / 1e9, 1),
+    }
+    result["device_stream"]["gather_vs_read"] = round(achieved / result["device_stream"]["read_GB/s"], 3)
+    log(f"device streaming rates: {result['device_stream']}")
+
     if not args.no_extras and args.mode == "fused":
         # plain two-stage form on the same plan, for comparison
         tf.zero_()

@@ -112,6 +112,7 @@ PROTOTYPES = {
                              _i64, _p64, _vp]),
     "g2g_optimal_reordering": (_i, [ctypes.POINTER(G2GEdge), _i64, ctypes.c_int32, _p32, _p32]),
     "g2g_fill_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, _vp]),
+    "g2g_stream_probe": (_i, [_i, _vp, _vp, _i64, _i, _vp, ctypes.POINTER(ctypes.c_double)]),
     "g2g_verify_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, ctypes.c_char, _i64, _vp,
                                _p64]),
 }

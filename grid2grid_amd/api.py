@@ -282,3 +282,13 @@ def optimal_reordering(volume, n_ranks):
     reordered = ctypes.c_int32()
     check(lib.g2g_optimal_reordering(edges, len(items), n_ranks, perm, ctypes.byref(reordered)))
     return [int(x) for x in perm], bool(reordered.value)
+
+
+def stream_probe(kind, src, dst, nbytes, reps=10, stream=None):
+    """Streaming reference rate of the device: kind 0 copy src -> dst, 1 read
+    src, 2 write dst; returns milliseconds per pass over nbytes (see
+    g2g_stream_probe)."""
+    ms = ctypes.c_double()
+    check(_lib.load().g2g_stream_probe(int(kind), ctypes.c_void_p(src or None), ctypes.c_void_p(dst or None),
+                                       int(nbytes), int(reps), _stream(stream), ctypes.byref(ms)))
+    return ms.value

@@ -1317,4 +1317,28 @@ int g2g_verify_layout(const g2g_layout *final_, int dtype, uint64_t seed, char t
     return G2G_OK;
 }
 
+int g2g_stream_probe(int kind, const void *src, void *dst, int64_t bytes, int reps, void *stream,
+                     double *ms) {
+    if (kind < 0 || kind > 2 || !ms || reps < 1 || bytes <= 0 || (bytes & 15) || !dst || (kind != 2 && !src) ||
+        ((uintptr_t)src & 15) || ((uintptr_t)dst & 15))
+        return fail(G2G_ERR_INVALID, "stream probe: kind 0-2, 16-byte aligned buffers, bytes > 0 and a multiple of 16");
+    hipStream_t s = (hipStream_t)stream;
+    const int grid = device_cus() * 16;
+    hipEvent_t e0, e1;
+    G2G_HIP(hipEventCreate(&e0));
+    G2G_HIP(hipEventCreate(&e1));
+    hipError_t he = launch_stream(kind, src, dst, (uint64_t)bytes, grid, s);  // warm-up
+    if (he == hipSuccess) he = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && he == hipSuccess; ++r) he = launch_stream(kind, src, dst, (uint64_t)bytes, grid, s);
+    if (he == hipSuccess) he = hipEventRecord(e1, s);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float t = 0;
+    if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (he != hipSuccess) return fail(G2G_ERR_HIP, "stream probe: %s", hipGetErrorString(he));
+    *ms = (double)t / reps;
+    return G2G_OK;
+}
+
 }  // extern "C"

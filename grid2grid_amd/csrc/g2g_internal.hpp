@@ -239,5 +239,8 @@ hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, ui
 hipError_t launch_verify(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
                          int trans, int conj, int64_t a_n_cols, unsigned long long *bad,
                          hipStream_t stream);
+// Streaming copy (kind 0), read (1) or write (2) of `bytes` (a multiple of
+// 16; src and dst 16-byte aligned) with `grid` workgroups.
+hipError_t launch_stream(int kind, const void *src, void *dst, uint64_t bytes, int grid, hipStream_t stream);
 
 }  // namespace g2g

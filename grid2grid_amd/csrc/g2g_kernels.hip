@@ -1509,6 +1509,44 @@ hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, ui
     return hipGetLastError();
 }
 
+// Streaming reference rates of the device (test support): each thread moves
+// 8 16-byte vectors per grid stride with non-temporal hints; copy reads
+// src and writes dst, read only reads (folding into one value that is
+// written only if it matches a constant), write only writes. No scratch,
+// every access below n vectors.
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_stream(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                     uint64_t n) {
+    constexpr int U = 8;
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads * U;
+    u32x4 acc = {0u, 0u, 0u, 0u};
+    const u32x4 fill = {0x3ff00000u, 0u, 0x3ff00000u, threadIdx.x};
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads * U + threadIdx.x; i < n; i += stride) {
+        u32x4 r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (KIND != 2 && i + (uint64_t)u * kThreads < n) r[u] = __builtin_nontemporal_load(src + i + u * kThreads);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i + (uint64_t)u * kThreads >= n) continue;
+            if (KIND == 0) __builtin_nontemporal_store(r[u], dst + i + u * kThreads);
+            if (KIND == 1) acc ^= r[u];
+            if (KIND == 2) __builtin_nontemporal_store(fill, dst + i + u * kThreads);
+        }
+    }
+    if (KIND == 1 && (acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9e3779b9u && n) dst[0] = acc;
+}
+
+hipError_t launch_stream(int kind, const void *src, void *dst, uint64_t bytes, int grid, hipStream_t stream) {
+    const uint64_t n = bytes / 16;
+    const u32x4 *s = (const u32x4 *)src;
+    u32x4 *d = (u32x4 *)dst;
+    if (kind == 0) hipLaunchKernelGGL(k_stream<0>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
+    else if (kind == 1) hipLaunchKernelGGL(k_stream<1>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
+    else hipLaunchKernelGGL(k_stream<2>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_verify(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
                          int trans, int conj, int64_t a_n_cols, unsigned long long *bad,
                          hipStream_t stream) {

@@ -351,6 +351,15 @@ int g2g_fill_layout(const g2g_layout *layout, int dtype, uint64_t seed, void *st
 int g2g_verify_layout(const g2g_layout *final, int dtype, uint64_t seed, char trans,
                       int64_t a_n_cols, void *stream, int64_t *mismatches);
 
+/* Streaming reference rate of the device, for context beside a transform's
+ * bandwidth: `bytes` (> 0, a multiple of 16; buffers 16-byte aligned) copied
+ * from src to dst (kind 0), read from src (1; dst receives at most 16 bytes)
+ * or written to dst (2), by 16 workgroups per CU moving 16-byte vectors with
+ * non-temporal hints; average milliseconds over `reps` launches after one
+ * warm-up (hipEvents on `stream`) in *ms. Overwrites dst. */
+int g2g_stream_probe(int kind, const void *src, void *dst, int64_t bytes, int reps, void *stream,
+                     double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -450,3 +450,24 @@ def test_beyond_32_bit_element_indices():
     plan.close()
     del ti, tf
     torch.cuda.empty_cache()
+
+
+def test_stream_probe():
+    """The streaming reference passes: the copy copies, the write writes its
+    pattern, the read leaves the source alone; bad arguments are refused."""
+    n = 64 << 20
+    src = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
+    keep = src.clone()
+    dst = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    assert g2g.stream_probe(0, src.data_ptr(), dst.data_ptr(), n, reps=2) > 0
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    assert g2g.stream_probe(1, src.data_ptr(), dst.data_ptr(), n, reps=2) > 0
+    assert g2g.stream_probe(2, src.data_ptr(), dst.data_ptr(), n, reps=2) > 0
+    torch.cuda.synchronize()
+    assert torch.equal(src, keep)
+    words = dst.view(torch.int32).view(-1, 4)[:1024].cpu().numpy()
+    assert (words[:, 0] == 0x3ff00000).all() and (words[:, 1] == 0).all()
+    for bad in ((3, n), (0, n + 4), (0, 0)):
+        with pytest.raises(g2g.G2GError):
+            g2g.stream_probe(bad[0], src.data_ptr(), dst.data_ptr(), bad[1])

@@ -255,6 +255,7 @@ struct g2g_plan_s {
     stage_table chunk;               // column-chunked pieces (gather stream when use_chunk)
     bool use_chunk = false;
     uint32_t chunk_w = 0;            // columns per chunk
+    uint64_t stream_gen = 0;         // changes whenever the gather's piece table changes
     uint64_t class_key = 0;          // shape class of the tuning table
     double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
     dbuf<uint32_t> ranks;            // (src << 16) | dst per piece
@@ -615,13 +616,16 @@ uint64_t chunk_count(const g2g_plan_s *p, uint32_t w) {
 // merged in L2 instead of reaching memory partly written, twice).
 int select_chunks(g2g_plan p, uint32_t w, hipStream_t s) {
     if (w == 0 || p->fused.n == 0) {
+        if (p->use_chunk) ++p->stream_gen;
         p->use_chunk = false;
         return G2G_OK;
     }
     if (p->chunk.n && p->chunk_w == w) {
+        if (!p->use_chunk) ++p->stream_gen;
         p->use_chunk = true;
         return G2G_OK;
     }
+    ++p->stream_gen;  // batches built on the old table must not run it
     p->use_chunk = false;
     p->chunk.pieces.release();
     p->chunk.pend.release();
@@ -1000,6 +1004,7 @@ struct g2g_batch_s {
     uint64_t tile_bytes = 0;
     launch_cfg cfg;
     double tuned_ms = 0;
+    std::vector<uint64_t> plan_gen;     // each plan's stream generation when its desc was made
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void drop_graph() {
@@ -1013,7 +1018,18 @@ struct g2g_batch_s {
 
 namespace {
 
+// A batch holds its plans' piece tables by address: a plan retuned to
+// another stream since (g2g_plan_set_tuning, g2g_plan_autotune) has freed or
+// left the table the batch would read.
+int batch_check(g2g_batch b) {
+    for (size_t k = 0; k < b->plans.size(); ++k)
+        if (b->plans[k]->stream_gen != b->plan_gen[k])
+            return fail(G2G_ERR_INVALID, "plan %zu was retuned after the batch was built: build the batch again", k);
+    return G2G_OK;
+}
+
 int batch_launch(g2g_batch b, hipStream_t s) {
+    G2G_TRY(batch_check(b));
     gather_args a;
     std::memset(&a, 0, sizeof a);
     a.descs = b->descs.p;
@@ -1137,6 +1153,7 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
     uint32_t tiles = 0;
     G2G_TRY(internal_descs(n, plans, alphas, betas, b->host_descs, tiles, b->total_bytes));
     b->plans.assign(plans, plans + n);
+    for (int k = 0; k < n; ++k) b->plan_gen.push_back(plans[k]->stream_gen);
     G2G_TRY(b->queue.alloc(kQueueWords));
     // batches always run persistent; g2g_set_tuning may ask for per-XCD ranges
     set_launch_mode(b->cfg, g_launch_mode.load() == G2G_LAUNCH_PERSISTENT_XCD ? G2G_LAUNCH_PERSISTENT_XCD
@@ -1157,6 +1174,7 @@ int g2g_batch_execute_graph(g2g_batch b, void *stream) {
     if (!b) return fail(G2G_ERR_INVALID, "batch is NULL");
     hipStream_t s = (hipStream_t)stream;
     if (!s) return fail(G2G_ERR_INVALID, "graph replay needs a non-null stream");
+    G2G_TRY(batch_check(b));
     if (!b->exec) {
         G2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int rc = batch_launch(b, s);

@@ -176,7 +176,9 @@ typedef struct g2g_batch_s *g2g_batch;
  * and transformer<T> (transform.cpp:166-204, transformer.hpp:8-62), without
  * their single-T restriction. alphas[k]/betas[k] point to one element of plan
  * k's dtype, or the arrays are NULL (plain copies). Plans must outlive the
- * batch. */
+ * batch. The batch runs each plan's stream as tuned at creation: after a plan
+ * is retuned (g2g_plan_set_tuning, g2g_plan_autotune) the batch's launches
+ * fail with G2G_ERR_INVALID until it is built again. */
 int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
                      const void *const *betas, void *stream, g2g_batch *out);
 

@@ -153,3 +153,32 @@ def test_server_plain_mode_and_budget_reject():
     assert e.value.code == 4
     assert small.stats()["rejected"] == 1
     small.close()
+
+
+def test_batch_refuses_after_a_plan_is_retuned():
+    """A batch holds its plans' piece tables by address: once a plan is
+    retuned to another stream (column chunks), launching the old batch fails
+    cleanly, and a batch built again gives every request's own result."""
+    cases = _cases(6, 9)
+    untransposed = [c for c in cases if c.trans == "N"]
+    assert untransposed
+    batch = g2g.Batch([c.plan for c in cases])
+    batch.execute()
+    torch.cuda.synchronize()
+    t = untransposed[0].plan.tuning
+    untransposed[0].plan.set_tuning(t["tile_bytes"], t["threads"], t["unroll_bytes"], t["nontemporal"],
+                                    t["launch_mode"], column_chunks=3)
+    with pytest.raises(g2g.G2GError):
+        batch.execute()
+    s = torch.cuda.Stream()
+    with pytest.raises(g2g.G2GError):
+        batch.execute_graph(s)
+    batch.close()
+    for c in cases:
+        c.reset()
+    batch = g2g.Batch([c.plan for c in cases])
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, c in enumerate(cases):
+        c.check(f"rebuilt batch request {k}")
+    batch.close()

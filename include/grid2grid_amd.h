@@ -234,7 +234,8 @@ int g2g_server_create(int64_t budget_bytes, int n_streams, int max_batch, int mo
 
 /* Queue plan for execution (alpha/beta as for g2g_transform, copied).
  * Returns a ticket; G2G_ERR_BUDGET if the request can never fit the budget.
- * A plan must not be submitted again before its previous ticket completed. */
+ * A plan must not be submitted again, retuned (g2g_plan_set_tuning,
+ * g2g_plan_autotune) or destroyed before its previous ticket completed. */
 int g2g_server_submit(g2g_server s, g2g_plan plan, const void *alpha, const void *beta,
                       int64_t *ticket);
 

@@ -113,6 +113,14 @@ class transform_plan {
         detail::check(g2g_plan_get_info(p_, &i));
         return i;
     }
+    // Times the compiled variants on this layout pair (the final layout is
+    // written meanwhile) and keeps the fastest; plans created later for the
+    // same shape class start from it (g2g_plan_autotune).
+    g2g_tuning autotune(void *stream = nullptr, int reps = 3) {
+        g2g_tuning t;
+        detail::check(g2g_plan_autotune(p_, stream, reps, &t));
+        return t;
+    }
     g2g_plan handle() const { return p_; }
 
   private:

@@ -57,7 +57,10 @@ struct rank_buffers {
     }
 };
 
-static bool run_case(int m, int n, int ibm, int ibn, int fbm, int fbn, int pm, int pn, char trans) {
+// tuned: the layout pair is planned once (transform_plan), autotuned, and
+// run again, then transform() runs a fresh plan of the same shape class
+static bool run_case(int m, int n, int ibm, int ibn, int fbm, int fbn, int pm, int pn, char trans,
+                     bool tuned = false) {
     const int P = pm * pn;
     scalapack::data_layout l1({m, n}, {ibm, ibn}, {pm, pn}, scalapack::column_major);
     const int fm = trans == 'N' ? m : n, fn = trans == 'N' ? n : m;
@@ -80,26 +83,44 @@ static bool run_case(int m, int n, int ibm, int ibn, int fbm, int fbn, int pm, i
         per2.push_back(get_scalapack_grid<double>(l2, d2, r));
     }
     grid_layout<double> init = merge_ranks(per1), fin = merge_ranks(per2);
+    bool ok = true;
+    auto verify = [&]() {
+        for (int r = 0; r < P; ++r) {
+            std::vector<double> got(b2.size[r]);
+            HIPCHECK(hipMemcpy(got.data(), b2.dev[r], sizeof(double) * got.size(), hipMemcpyDeviceToHost));
+            std::vector<double> want = local_values(l2, r, [&](int i, int j) {
+                return trans == 'N' ? cantor(i, j) : cantor(j, i);
+            });
+            for (size_t k = 0; k < got.size(); ++k) {
+                // padding of the padded local buffers is never written: keeps -1
+                const double w = want[k] == -7.0 ? -1.0 : want[k];
+                if (got[k] != w) {
+                    if (ok) std::printf("rank %d element %zu: %g instead of %g\n", r, k, got[k], w);
+                    ok = false;
+                }
+            }
+        }
+    };
+    auto reset = [&]() {  // the final buffers back to the sentinel values
+        for (int r = 0; r < P; ++r)
+            HIPCHECK(hipMemcpy(b2.dev[r], std::vector<double>(b2.size[r], -1.0).data(), sizeof(double) * b2.size[r],
+                               hipMemcpyHostToDevice));
+    };
+    if (tuned) {
+        transform_plan<double> plan(init, fin, trans);
+        const g2g_tuning t = plan.autotune(nullptr, 1);
+        std::printf("autotuned: %lld-byte work items, %d threads, column chunks %d\n", (long long)t.tile_bytes,
+                    t.threads, t.column_chunks);
+        reset();
+        plan.execute();
+        verify();
+        reset();
+    }
     if (trans == 'N')
         transform<double>(init, fin);
     else
         transform<double>(init, fin, trans, 1.0, 0.0);
-    bool ok = true;
-    for (int r = 0; r < P; ++r) {
-        std::vector<double> got(b2.size[r]);
-        HIPCHECK(hipMemcpy(got.data(), b2.dev[r], sizeof(double) * got.size(), hipMemcpyDeviceToHost));
-        std::vector<double> want = local_values(l2, r, [&](int i, int j) {
-            return trans == 'N' ? cantor(i, j) : cantor(j, i);
-        });
-        for (size_t k = 0; k < got.size(); ++k) {
-            // padding of the padded local buffers is never written: keeps -1
-            const double w = want[k] == -7.0 ? -1.0 : want[k];
-            if (got[k] != w) {
-                if (ok) std::printf("rank %d element %zu: %g instead of %g\n", r, k, got[k], w);
-                ok = false;
-            }
-        }
-    }
+    verify();
     std::printf("m=%d n=%d init %dx%d final %dx%d grid %dx%d trans %c: %s\n", m, n, ibm, ibn, fbm, fbn, pm, pn,
                 trans, ok ? "correct" : "NOT correct");
     return ok;
@@ -294,6 +315,7 @@ int main(int argc, char **argv) {
                              {10, 12, 2, 3, 2, 6, 2, 2}, {10, 12, 2, 3, 3, 5, 2, 2}};
     for (const auto &c : cases) ok &= run_case(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], 'N');
     ok &= run_case(64, 48, 8, 8, 5, 7, 2, 2, 'T');
+    ok &= run_case(700, 500, 24, 40, 56, 32, 2, 2, 'N', true);
     ok &= transformer_case();
     ok &= volume_case();
     std::printf("%s\n", ok ? "ALL CORRECT" : "FAILURES");

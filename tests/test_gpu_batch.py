@@ -182,3 +182,32 @@ def test_batch_refuses_after_a_plan_is_retuned():
     for k, c in enumerate(cases):
         c.check(f"rebuilt batch request {k}")
     batch.close()
+
+
+def test_batch_of_the_benchmark_distribution():
+    """512 requests drawn from the benchmark's mixed distribution at full
+    size (m, n up to 4096, f32/f64/c64, blocks 16..256, grids up to 4x4, 30%
+    transposed) in one hipGraph-replayed launch: after each of two replays
+    every request's final layout equals the seeded closed form."""
+    reqs = workloads.cfg4(n=512, seed=77)
+    rs = []
+    for k, (init, fin, dtype, trans) in enumerate(reqs):
+        es = g2g.DTYPE_SIZE[dtype]
+        ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device="cuda")
+        tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device="cuda")
+        g2g.fill_layout(init, ti.data_ptr(), dtype, 300 + k)
+        rs.append((init, fin, dtype, trans, ti, tf, g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)))
+    s = torch.cuda.Stream()
+    batch = g2g.Batch([r[6] for r in rs], stream=s)
+    for rep in range(2):
+        for r in rs:
+            r[5].zero_()
+        torch.cuda.synchronize()
+        batch.execute_graph(s)
+        s.synchronize()
+        bad = [k for k, (init, fin, dtype, trans, ti, tf, plan) in enumerate(rs)
+               if g2g.verify_layout(fin, tf.data_ptr(), dtype, 300 + k, trans, init.n) != 0]
+        assert not bad, f"replay {rep}: requests {bad[:10]} differ"
+    batch.close()
+    for r in rs:
+        r[6].close()

@@ -471,3 +471,34 @@ def test_stream_probe():
     for bad in ((3, n), (0, n + 4), (0, 0)):
         with pytest.raises(g2g.G2GError):
             g2g.stream_probe(bad[0], src.data_ptr(), dst.data_ptr(), bad[1])
+
+
+@pytest.mark.parametrize("cfg,chunks", [("cfg1", 0), ("cfg2", 0), ("cfg2", 16), ("cfg3", 0), ("cfg3", 8)])
+def test_benchmark_configurations_full_size(cfg, chunks):
+    """The benchmark's own layout pairs at full size: 4096^2 f64, 32768^2 f64
+    (8 GiB a side) and the irregular 40000x25000 f32 with padded, shuffled
+    blocks, on the pieces and on a column-chunked stream. Every element of the
+    final layout equals the seeded value of its source element (checked on the
+    device), and no byte outside the final blocks (ld padding, gaps) is
+    touched."""
+    from grid2grid_amd import workloads
+    init, fin, dtype, trans = getattr(workloads, cfg)()
+    es = g2g.DTYPE_SIZE[dtype]
+    ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device="cuda")
+    tf = torch.full((fin.arena_elems * es,), 0xA5, dtype=torch.uint8, device="cuda")
+    g2g.fill_layout(init, ti.data_ptr(), dtype, 11)
+    plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+    t = plan.tuning
+    plan.set_tuning(t["tile_bytes"], 512, 64, 1, 1, column_chunks=chunks)
+    print(f"{cfg} chunks {chunks}: {plan.info.n_pieces} pieces, {plan.info.n_tiles} work items", flush=True)
+    plan.transform()
+    torch.cuda.synchronize()
+    assert g2g.verify_layout(fin, tf.data_ptr(), dtype, 11, trans, init.n) == 0
+    mask = fin.element_mask()
+    if not mask.all():
+        inside = torch.from_numpy(mask).to("cuda")
+        changed = (tf.view(-1, es) != 0xA5).any(dim=1)
+        assert not bool((changed & ~inside).any()), "a byte outside the final blocks was written"
+    plan.close()
+    del ti, tf
+    torch.cuda.empty_cache()

@@ -703,9 +703,10 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     const int64_t np = n_ur * n_uc;
     // piece indices, plus the 64 examined at once by a wave, stay within 32 bits
     if (np > 0xffffff00ll) return fail(G2G_ERR_UNSUPPORTED, "too many pieces (%lld)", (long long)np);
+    // the gather rounds a column's bytes up to 16 in 32-bit arithmetic
     for (const span &r : p->urows)
-        if ((uint64_t)(r.end - r.start) * es > 0xffffffffull)
-            return fail(G2G_ERR_UNSUPPORTED, "piece column exceeds 4 GiB");
+        if ((uint64_t)(r.end - r.start) * es > 0xffffffffull - 32)
+            return fail(G2G_ERR_UNSUPPORTED, "piece column exceeds 4 GiB - 32 bytes");
     dbuf<span> d_ur, d_uc;
     dbuf<int32_t> d_irs, d_ics, d_frs, d_fcs;
     dbuf<cell> d_ic, d_fc;

@@ -214,7 +214,9 @@ __device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::ty
 // Runs queued for one flat copy.
 // SEG_VEC: columns of colb bytes starting at s (source, leading dimension
 //   lds_b bytes) and d (destination, ldd_b), each cut into vpc = ceil(colb/16)
-//   16-byte slots, the last one partial when colb is not a multiple of 16.
+//   16-byte slots; when colb is not a multiple of 16 the last slot is the
+//   column's last 16 bytes (overlapping the slot before it), or, in a column
+//   shorter than 16 bytes, its 1-3 dwords.
 // SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece of
 //   ncols final columns, one slot per final row and group of 16/es final columns
 //   (vpc = final rows): destination element (i, c) at d + c*ldd_b + i*es comes
@@ -253,7 +255,11 @@ __device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t m
 // threads in order, so consecutive lanes move consecutive 16-byte vectors of a
 // column (a wave spans several short columns, or one stretch of a long one,
 // and several pieces), and every thread keeps UB bytes in flight whatever the
-// piece shapes. Addresses need only be dword-aligned.
+// piece shapes. Addresses need only be dword-aligned. Every slot of a column
+// of 16 bytes or more is one 16-byte load and one 16-byte store: the last slot
+// of a ragged column rewrites up to 12 bytes of the slot before it with the
+// same values (source and destination do not overlap), so no wave takes a
+// dword-by-dword path for the columns' tails.
 template <int THREADS, int UB, bool NT>
 __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total) {
     constexpr int U = UB / 16;
@@ -281,10 +287,13 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
                 const uint32_t rel = k - e.slot0;
                 const uint32_t c = magic_div(rel, e.vpc, e.vmul);
                 const uint32_t v = rel - c * e.vpc;
-                const gbyte *sp = (const gbyte *)e.s + c * e.lds_b + (uint64_t)v * 16;
-                dp[u] = (gbyte *)e.d + c * e.ldd_b + (uint64_t)v * 16;
-                // dwords in this slot: 4, or 1-3 for the last slot of a ragged column
-                tail[u] = (v + 1 == e.vpc && (e.colb & 15)) ? (e.colb & 15) >> 2 : 4;
+                // column byte offset of the slot (v*16 < colb: no wrap)
+                uint32_t off = v * 16;
+                if (e.colb >= 16 && off > e.colb - 16) off = e.colb - 16;
+                const gbyte *sp = (const gbyte *)e.s + c * e.lds_b + off;
+                dp[u] = (gbyte *)e.d + c * e.ldd_b + off;
+                // dwords in this slot: 4, or 1-3 in a column shorter than 16 bytes
+                tail[u] = e.colb < 16 ? e.colb >> 2 : 4;
                 if (tail[u] == 4) {
                     r[u] = ld_a4<NT>(sp);
                 } else {
@@ -1036,8 +1045,11 @@ hipError_t launch_gather(const gather_args &a_in, const launch_cfg &c, int cus, 
     if (a_in.total_tiles == 0) return hipSuccess;
     gather_args a = a_in;
     a.xcd_ranges = c.xcd_ranges ? 1u : 0u;
-    // persistent grids fill every CU at full occupancy (32 waves per CU)
-    const int per_cu = c.threads >= 512 ? 4 : 8;
+    // persistent grids: 16 waves per CU (two 512-thread or four 256-thread
+    // workgroups); a third resident 512-thread workgroup, which the copy-only
+    // kernels' register counts allow, made the 32768^2 f64 block-cyclic
+    // transform ~5 % slower
+    const int per_cu = c.threads >= 512 ? 2 : 4;
     const int gl = cus * per_cu;
     return c.persistent ? launch_th<true>(a, c, gl, stream) : launch_th<false>(a, c, gl, stream);
 }

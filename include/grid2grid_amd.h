@@ -118,7 +118,8 @@ int g2g_synchronize(void *stream);
  * its storage orientation, `final` describes op(A)). Unlike the reference the
  * caller's `initial` descriptor is not modified.
  * Planning work is ordered on `stream`; the call returns after it finished.
- * The plan keeps the block pointers; the buffers must outlive the plan. */
+ * The plan keeps the block pointers; the buffers must outlive the plan, and
+ * the initial and final blocks must not overlap in memory. */
 int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final, int dtype, char trans,
                     void *stream, g2g_plan *out);
 

@@ -7,7 +7,7 @@ run by the fused gather kernel. One step = one full transform of the matrix.
 value = moved GB/s = 2*m*n*sizeof(T) / device time per step (one read and one
 write per element; inputs resident in HBM). Output: one JSON line.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bc32768]
                        [--mode fused|staged] [--no-cpu-baseline] [--no-extras]
 """
 import argparse
@@ -30,8 +30,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2",
-                    help="cfg1 | cfg2 | cfg3 | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
+    ap.add_argument("--workload", default="bc32768",
+                    help="bc4096 | bc32768 | irregular | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
                     help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode"
@@ -72,13 +72,13 @@ def build_workload(name):
         fm, fn = (n, m) if trans in "TC" else (m, n)
         return (scalapack_layout(m, n, imb, inb, ipr, ipc, "C", lld="rows"),
                 scalapack_layout(fm, fn, fmb, fnb, fpr, fpc, "C", lld="rows"), dtype, trans)
-    return {"cfg1": workloads.cfg1, "cfg2": workloads.cfg2, "cfg3": workloads.cfg3}[name]()
+    return {"bc4096": workloads.bc4096, "bc32768": workloads.bc32768, "irregular": workloads.irregular}[name]()
 
 
 WORKLOAD_DESC = {
-    "cfg1": "block-cyclic 4096^2 f64, 32x32 on 2x2 -> 64x128 on 2x2, 4 virtual ranks",
-    "cfg2": "block-cyclic 32768^2 f64, 128x128 on 4x4 -> 256x512 on 2x8, 16 virtual ranks",
-    "cfg3": "irregular 40000x25000 f32, 64x48 -> 50x40 splits, 32 virtual ranks, padded ld",
+    "bc4096": "block-cyclic 4096^2 f64, 32x32 on 2x2 -> 64x128 on 2x2, 4 virtual ranks",
+    "bc32768": "block-cyclic 32768^2 f64, 128x128 on 4x4 -> 256x512 on 2x8, 16 virtual ranks",
+    "irregular": "irregular 40000x25000 f32, 64x48 -> 50x40 splits, 32 virtual ranks, padded ld",
     "copy": "single-block 32768^2 f64 -> single block: plain contiguous 8 GiB copy (calibration)",
 }
 
@@ -131,7 +131,7 @@ def cpu_baseline(budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from grid2grid_amd import workloads
-    init, fin, dtype, _ = workloads.cfg2(m=8192)
+    init, fin, dtype, _ = workloads.bc32768(m=8192)
     es = O.ES[dtype]
     ia = np.zeros(init.arena_elems * es, dtype=np.uint8)
     fa = np.zeros(fin.arena_elems * es, dtype=np.uint8)
@@ -160,7 +160,7 @@ def cpu_baseline(budget_s):
         "unit": "GB/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"8192^2 f64, config-2 blocks/grids (128x128 on 4x4 -> 256x512 on 2x8), 16 virtual ranks "
+        "sample": f"8192^2 f64, the default workload's blocks/grids (128x128 on 4x4 -> 256x512 on 2x8), 16 virtual ranks "
                   f"in one process, median of {len(times)} runs after 2 warm-ups, {threads} OpenMP threads; "
                   f"time includes the reference's per-call planning",
         "ms_per_transform": round(med * 1e3, 3),
@@ -194,16 +194,16 @@ def verify_requests(g2g, rs):
 
 
 def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, tile_bytes=0):
-    """Config 4: n_req mixed transforms in one persistent launch (hipGraph
+    """Mixed requests: n_req transforms in one persistent launch (hipGraph
     replay) versus plain per-request launches (the staged form per request)
     and fused per-request launches."""
     t0 = time.time()
     if tile_bytes:
         g2g.set_tuning(tile_bytes, -1)
-    rs = make_requests(torch, g2g, workloads.cfg4(n=n_req, p_trans=p_trans), 40000, dev)
+    rs = make_requests(torch, g2g, workloads.mixed_requests(n=n_req, p_trans=p_trans), 40000, dev)
     g2g.set_tuning(0, -1)
     nbytes = sum(r["bytes"] for r in rs)
-    log(f"cfg4: {n_req} requests, {nbytes / 1e9:.2f} GB moved per pass, planned in {time.time() - t0:.1f}s")
+    log(f"mixed requests: {n_req} requests, {nbytes / 1e9:.2f} GB moved per pass, planned in {time.time() - t0:.1f}s")
     batch = g2g.Batch([r["plan"] for r in rs], stream=stream)
     t0 = time.time()
     btune = batch.autotune(stream, reps=2)
@@ -244,7 +244,7 @@ def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, t
 
 
 def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev):
-    """Config 5: Poisson arrivals at `qps` of config-4 requests plus 5% large
+    """Served stream: Poisson arrivals at `qps` of mixed requests plus 5% large
     16384^2 f64 requests, served with continuous batching under an HBM budget,
     versus plain per-request launches; then the saturated throughput of both."""
     import numpy as np
@@ -252,7 +252,7 @@ def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev)
     t0 = time.time()
     small = make_requests(torch, g2g, [workloads.random_request(rng) for _ in range(n_small)], 50000, dev)
     large = make_requests(torch, g2g, [workloads.large_request() for _ in range(n_large)], 60000, dev)
-    log(f"cfg5: pool of {n_small} small + {n_large} large requests ready in {time.time() - t0:.1f}s")
+    log(f"served stream: pool of {n_small} small + {n_large} large requests ready in {time.time() - t0:.1f}s")
     for r in small + large:  # staging buffers of the plain form, outside the timed region
         r["plan"].transform_staged()
     torch.cuda.synchronize()
@@ -314,7 +314,7 @@ def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev)
         log(f"{name}: {res[name]}")
     bad = verify_requests(g2g, small + large)
     out = {
-        "workload": f"Poisson {qps} req/s, {n_req} requests (config-4 distribution + 5% 16384^2 f64), "
+        "workload": f"Poisson {qps} req/s, {n_req} requests (mixed-request distribution + 5% 16384^2 f64), "
                     f"HBM budget {budget / 1e9:.0f} GB",
         **res,
         "throughput_ratio_saturated": round(res["served_batched"]["saturated_GB/s"] /

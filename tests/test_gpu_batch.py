@@ -189,7 +189,7 @@ def test_batch_of_the_benchmark_distribution():
     size (m, n up to 4096, f32/f64/c64, blocks 16..256, grids up to 4x4, 30%
     transposed) in one hipGraph-replayed launch: after each of two replays
     every request's final layout equals the seeded closed form."""
-    reqs = workloads.cfg4(n=512, seed=77)
+    reqs = workloads.mixed_requests(n=512, seed=77)
     rs = []
     for k, (init, fin, dtype, trans) in enumerate(reqs):
         es = g2g.DTYPE_SIZE[dtype]

@@ -33,7 +33,7 @@ def _layout_pairs():
     return {
         "bc_small": (scalapack_layout(37, 29, 4, 3, 2, 2, "C"), scalapack_layout(37, 29, 5, 7, 2, 2, "C")),
         "bc_grid_change": (scalapack_layout(100, 77, 8, 8, 2, 3, "R"), scalapack_layout(100, 77, 3, 5, 3, 1, "C")),
-        "bc_cfg1_shape": (scalapack_layout(256, 256, 32, 32, 2, 2, "C", lld="rows"),
+        "bc_4096_blocks": (scalapack_layout(256, 256, 32, 32, 2, 2, "C", lld="rows"),
                           scalapack_layout(256, 256, 64, 128, 2, 2, "C", lld="rows")),
         "arbitrary": (arbitrary_layout(rng, 300, 260, 7, 6, 5, lo=10, hi=80, pad=(0, 9), gap=(0, 33)),
                       arbitrary_layout(rng, 300, 260, 5, 8, 6, lo=10, hi=80, pad=(0, 9), gap=(0, 33))),
@@ -473,8 +473,9 @@ def test_stream_probe():
             g2g.stream_probe(bad[0], src.data_ptr(), dst.data_ptr(), bad[1])
 
 
-@pytest.mark.parametrize("cfg,chunks", [("cfg1", 0), ("cfg2", 0), ("cfg2", 16), ("cfg3", 0), ("cfg3", 8)])
-def test_benchmark_configurations_full_size(cfg, chunks):
+@pytest.mark.parametrize("name,chunks", [("bc4096", 0), ("bc32768", 0), ("bc32768", 16), ("irregular", 0),
+                                         ("irregular", 8)])
+def test_benchmark_workloads_full_size(name, chunks):
     """The benchmark's own layout pairs at full size: 4096^2 f64, 32768^2 f64
     (8 GiB a side) and the irregular 40000x25000 f32 with padded, shuffled
     blocks, on the pieces and on a column-chunked stream. Every element of the
@@ -482,7 +483,7 @@ def test_benchmark_configurations_full_size(cfg, chunks):
     device), and no byte outside the final blocks (ld padding, gaps) is
     touched."""
     from grid2grid_amd import workloads
-    init, fin, dtype, trans = getattr(workloads, cfg)()
+    init, fin, dtype, trans = getattr(workloads, name)()
     es = g2g.DTYPE_SIZE[dtype]
     ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device="cuda")
     tf = torch.full((fin.arena_elems * es,), 0xA5, dtype=torch.uint8, device="cuda")
@@ -490,7 +491,7 @@ def test_benchmark_configurations_full_size(cfg, chunks):
     plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
     t = plan.tuning
     plan.set_tuning(t["tile_bytes"], 512, 64, 1, 1, column_chunks=chunks)
-    print(f"{cfg} chunks {chunks}: {plan.info.n_pieces} pieces, {plan.info.n_tiles} work items", flush=True)
+    print(f"{name} chunks {chunks}: {plan.info.n_pieces} pieces, {plan.info.n_tiles} work items", flush=True)
     plan.transform()
     torch.cuda.synchronize()
     assert g2g.verify_layout(fin, tf.data_ptr(), dtype, 11, trans, init.n) == 0

@@ -249,6 +249,41 @@ def test_work_item_sizes(tile_bytes, launch_mode):
     np.testing.assert_array_equal(got_t, wantT)
 
 
+@pytest.mark.parametrize("dtype", [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64])
+def test_ragged_column_tails(dtype):
+    """Pieces of 1-11 rows (4 to 88 bytes a column), blocks with 0-3 elements
+    of ld padding and gaps, both arenas filled with random bytes: the
+    copy queue moves a column whose byte length is not a multiple of 16 with
+    a last 16-byte slot that overlaps the one before it, and a column shorter
+    than 16 bytes dword by dword. Every byte of the final arena (padding and
+    gaps included) matches the oracle, for every work-item size, launch
+    mode and the column-chunked stream."""
+    rng = np.random.default_rng(515 + dtype)
+    init = arbitrary_layout(rng, 150, 70, 27, 5, 3, lo=1, hi=11, pad=(0, 3), gap=(0, 5))
+    fin = arbitrary_layout(rng, 150, 70, 23, 6, 4, lo=1, hi=11, pad=(0, 3), gap=(0, 5))
+    ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+    want = _expected(init, ia, fin, fa, dtype)
+    ti = _dev(ia)
+    plan = None
+    try:
+        for tb, th, ub, nt, mode, chunks in ((4096, 256, 64, 0, 0, 0), (4096, 512, 128, 1, 1, 0),
+                                             (65536, 512, 64, 1, 1, 0), (65536, 256, 128, 0, 2, 0),
+                                             (4096, 512, 64, 0, 3, 0), (4096, 512, 64, 1, 1, 8)):
+            print(f"dtype {dtype} tuning {tb}:{th}:{ub}:{nt}:{mode}:{chunks}", flush=True)
+            tf = _dev(fa)
+            plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, "N")
+            plan.set_tuning(tb, th, ub, nt, mode, column_chunks=chunks)
+            plan.transform()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(tf.cpu().numpy(), want)
+            plan.close()
+            plan = None
+    finally:
+        if plan is not None:
+            plan.close()
+    np.testing.assert_array_equal(ti.cpu().numpy(), ia)
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_transposed_whole_tiles(dtype):
     """Transposed pieces of 260 (and 264) final rows by 150 (and 146) final

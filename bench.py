@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
     ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched leg")
-    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     return ap.parse_args()

@@ -410,6 +410,10 @@ def main():
         },
         "bit_exact": bad == 0,
         "mismatches": int(bad),
+        # every launch of the second pass timed on its own (HIP events on the
+        # launch stream): the spread behind roofline.achieved, which uses the mean
+        "launch_ms": {"mean": round(per_launch_ms, 4), "median": round(statistics.median(per), 4),
+                      "min": round(min(per), 4), "max": round(max(per), 4), "launches": len(per)},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),

@@ -155,6 +155,12 @@ def cpu_baseline(budget_s):
         if (time.time() > t_end and len(times) >= 3) or len(times) >= 200:
             break
     med = statistics.median(times)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(nbytes / med / 1e9, 3),
         "unit": "GB/s",
@@ -164,6 +170,9 @@ def cpu_baseline(budget_s):
                   f"in one process, median of {len(times)} runs after 2 warm-ups, {threads} OpenMP threads; "
                   f"time includes the reference's per-call planning",
         "ms_per_transform": round(med * 1e3, 3),
+        "host": {"cpu_model": cpu_model, "nproc": os.cpu_count(),
+                 "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+                 "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")},
     }
 
 

@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
     ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched leg")
+    ap.add_argument("--batch-graph-only", action="store_true",
+                    help="batched leg without the per-request comparisons (for counter runs)")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)
@@ -202,7 +204,7 @@ def verify_requests(g2g, rs):
     return bad
 
 
-def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, tile_bytes=0):
+def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, tile_bytes=0, per_request=True):
     """Mixed requests: n_req transforms in one persistent launch (hipGraph
     replay) versus plain per-request launches (the staged form per request)
     and fused per-request launches."""
@@ -221,6 +223,14 @@ def batched_leg(torch, g2g, workloads, n_req, steps, stream, dev, p_trans=0.3, t
     with torch.cuda.stream(s):
         ms_b, _ = time_steps(torch, lambda: batch.execute_graph(s), steps, 2, s)
     bad_b = verify_requests(g2g, rs)
+    if not per_request:
+        batch.close()
+        out = {"bytes_per_pass": int(nbytes),
+               "batched_graph": {"ms": round(ms_b, 3), "GB/s": round(nbytes / ms_b / 1e6, 1), "bit_exact": bad_b == 0},
+               "batch_tuning": btune}
+        for r in rs:
+            r["plan"].close()
+        return out
     for r in rs:
         r["tf"].zero_()
     ms_f, _ = time_steps(torch, lambda: [r["plan"].transform(stream=stream) for r in rs], max(2, steps // 2), 1,
@@ -353,8 +363,9 @@ def main():
     stream = torch.cuda.current_stream()
 
     if args.only_batched:
-        out = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev, args.batch_ptrans,
-                          args.batch_tile_bytes)
+        out = batched_leg(torch, g2g, workloads, args.batch_requests, args.steps if args.batch_graph_only else 5,
+                          stream, dev, args.batch_ptrans, args.batch_tile_bytes,
+                          per_request=not args.batch_graph_only)
         print(json.dumps({"batched": out}), flush=True)
         return
 

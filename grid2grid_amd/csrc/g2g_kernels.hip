@@ -218,9 +218,11 @@ __device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::ty
 //   column's last 16 bytes (overlapping the slot before it), or, in a column
 //   shorter than 16 bytes, its 1-3 dwords.
 // SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece of
-//   ncols final columns, one slot per final row and group of 16/es final columns
-//   (vpc = final rows): destination element (i, c) at d + c*ldd_b + i*es comes
-//   from s + i*lds_b + c*es, its imaginary sign bit flipped for SEG_CONJ.
+//   ncols final columns: destination element (i, c) at d + c*ldd_b + i*es
+//   comes from s + i*lds_b + c*es, its imaginary sign bit flipped for
+//   SEG_CONJ. One slot per group of 16/es final columns and, for 4- and 8-byte
+//   elements, group of 16/es final rows (vpc = ceil(rows*es/16)), or final
+//   row (16-byte elements, vpc = final rows).
 // slot0 numbers the segment's first slot in its queue.
 enum : uint32_t { SEG_VEC = 0, SEG_TRANS = 1, SEG_CONJ = 2 };
 struct seg_entry {
@@ -234,15 +236,17 @@ struct seg_entry {
     uint32_t ncols;  // element segments: final columns of the piece
 };
 constexpr int kMaxSeg = 64;
-// Transposed pieces with fewer final rows, or narrower final column ranges
-// (bytes), than these go to the element queue instead of LDS tiles.
-#ifndef G2G_TRANS_ELEM_ROWS
-#define G2G_TRANS_ELEM_ROWS 64
+// Transposed pieces whose final columns hold fewer bytes than this (counting
+// at most 8 bytes an element: f32 below 128 rows, the other types below 64),
+// or whose final column ranges are narrower (bytes), go to the element queue
+// instead of LDS tiles.
+#ifndef G2G_TRANS_ELEM_ROWB
+#define G2G_TRANS_ELEM_ROWB 512
 #endif
 #ifndef G2G_TRANS_ELEM_COLB
 #define G2G_TRANS_ELEM_COLB 256
 #endif
-constexpr uint32_t kTransElemRows = G2G_TRANS_ELEM_ROWS;
+constexpr uint32_t kTransElemRowBytes = G2G_TRANS_ELEM_ROWB;
 constexpr uint32_t kTransElemColBytes = G2G_TRANS_ELEM_COLB;
 
 __device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t mul) {
@@ -314,6 +318,78 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
                 q[0] = r[u].x;
                 if (tail[u] > 1) q[1] = r[u].y;
                 if (tail[u] > 2) q[2] = r[u].z;
+            }
+        }
+    }
+}
+
+// Copy every slot of the queued SEG_TRANS / SEG_CONJ segments of 4- or
+// 8-byte elements as VE x VE register tiles, VE = 16/ES. A slot is a group of
+// VE final rows [h*VE, +VE) by a group of VE final columns [g*VE, +VE)
+// (seg_entry.vpc = the piece's row groups): VE 16-byte reads, one per final
+// row (VE consecutive elements of source storage column i), a transpose in
+// registers, and VE 16-byte writes, one per final column (VE consecutive
+// rows). Consecutive lanes take consecutive row groups of one column group, so
+// each write instruction stores one contiguous run per destination column.
+// A partial row group (rows not a multiple of VE) or column group (the
+// piece's last, seg_entry.ncols) moves element by element.
+template <int THREADS, bool NT, int ES>
+__device__ __forceinline__ void flat_copy_tiles(const seg_entry *segs, uint32_t total) {
+    static_assert(ES == 4 || ES == 8, "register tiles of 4- and 8-byte elements");
+    using T = typename bits_t<ES>::type;
+    constexpr uint32_t VE = 16 / ES;
+    uint32_t j = 0;
+    seg_entry e = segs[0];
+    uint32_t next0 = segs[1].slot0;
+    for (uint32_t k = threadIdx.x; k < total; k += (uint32_t)THREADS) {
+        if (next0 <= k) {  // walk the slot starts, then load the one entry needed
+            do {
+                ++j;
+                next0 = segs[j + 1].slot0;
+            } while (next0 <= k);
+            e = segs[j];
+        }
+        const uint32_t rel = k - e.slot0;
+        const uint32_t g = magic_div(rel, e.vpc, e.vmul);
+        const uint32_t h = rel - g * e.vpc;
+        const uint32_t c = g * VE, i0 = h * VE;
+        const uint32_t nc = min(VE, e.ncols - c);               // columns of the group
+        const uint32_t nr = min(VE, e.colb / (uint32_t)ES - i0);  // rows of the group
+        const bool conj = ES > 4 && e.kind == SEG_CONJ;
+        const gbyte *sp = (const gbyte *)e.s + (uint64_t)i0 * e.lds_b + (uint64_t)c * ES;
+        u32x4 in[VE];
+#pragma unroll
+        for (uint32_t y = 0; y < VE; ++y) {
+            in[y] = u32x4{0u, 0u, 0u, 0u};
+            if (y < nr) {
+                const gbyte *q = sp + (uint64_t)y * e.lds_b;
+                if (nc == VE) {
+                    in[y] = ld_a4<NT>(q);
+                } else {
+#pragma unroll
+                    for (uint32_t x = 0; x < VE; ++x)
+                        if (x < nc) vec_set<ES>(in[y], x, *(const GLOBAL T *)(q + x * ES));
+                }
+            }
+        }
+        gbyte *dp = (gbyte *)e.d + (uint64_t)c * e.ldd_b + (uint64_t)i0 * ES;
+#pragma unroll
+        for (uint32_t x = 0; x < VE; ++x) {
+            if (x >= nc) continue;
+            u32x4 out;
+#pragma unroll
+            for (uint32_t y = 0; y < VE; ++y) {
+                T v = vec_elem<ES>(in[y], x);
+                if (conj) v = conj_bits(v);
+                vec_set<ES>(out, y, v);
+            }
+            gbyte *q = dp + (uint64_t)x * e.ldd_b;
+            if (nr == VE) {
+                st_a4<NT>(q, out);
+            } else {
+#pragma unroll
+                for (uint32_t y = 0; y < VE; ++y)
+                    if (y < nr) st<NT>((GLOBAL T *)(q + y * ES), vec_elem<ES>(out, y));
             }
         }
     }
@@ -823,12 +899,16 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
                     kind = PK_DIRECT;
                 } else if (trans) {
                     // transposed pieces that would leave an LDS tile mostly
-                    // empty move element by element
-                    kind = (rows < kTransElemRows || (c1 - c0) * es < kTransElemColBytes) ? PK_ELEM : PK_DIRECT;
+                    // empty go to the element queue (register tiles)
+                    kind = ((uint64_t)rows * min(es, 8u) < kTransElemRowBytes ||
+                            (uint64_t)(c1 - c0) * es < kTransElemColBytes)
+                               ? PK_ELEM : PK_DIRECT;
                 } else {
                     kind = PK_VEC;
                 }
-                vpc = kind == PK_VEC ? (rows * es + 15) >> 4 : rows;
+                // copy slots: 16-byte vectors per column; element slots: groups
+                // of 16/es final rows (register tiles), or rows for 16-byte types
+                vpc = kind == PK_VEC ? (rows * es + 15) >> 4 : es < 16 ? (rows * es + 15) >> 4 : rows;
             }
         }
         // pieces past the work item come after every piece inside it
@@ -870,8 +950,8 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
             if (nseg) flat_copy<THREADS, UB, NT>(segs, total);
             if constexpr (FEAT != FEAT_COPY) {
                 if (ntseg) {
-                    if (es == 4) flat_copy_elems<THREADS, NT, 4>(tsegs, ttotal);
-                    else if (es == 8) flat_copy_elems<THREADS, NT, 8>(tsegs, ttotal);
+                    if (es == 4) flat_copy_tiles<THREADS, NT, 4>(tsegs, ttotal);
+                    else if (es == 8) flat_copy_tiles<THREADS, NT, 8>(tsegs, ttotal);
                     else flat_copy_elems<THREADS, NT, 16>(tsegs, ttotal);
                 }
             }

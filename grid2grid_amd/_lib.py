@@ -87,6 +87,12 @@ PROTOTYPES = {
     "g2g_plan_get_info": (_i, [_vp, ctypes.POINTER(G2GPlanInfo)]),
     "g2g_transform": (_i, [_vp, _vp, _vp, _vp]),
     "g2g_transform_staged": (_i, [_vp, _vp, _vp, _vp]),
+    "g2g_transform_graph": (_i, [_vp, _vp, _vp, _vp]),
+    "g2g_transform_layouts": (_i, [ctypes.POINTER(G2GLayout), ctypes.POINTER(G2GLayout), _i, ctypes.c_char,
+                                   _vp, _vp, _vp]),
+    "g2g_plan_cache_set_capacity": (_i, [_i64]),
+    "g2g_plan_cache_clear": (_i, []),
+    "g2g_plan_cache_stats": (_i, [_p64, _p64, _p64]),
     "g2g_plan_counts": (_i, [_vp, _p64, _i64]),
     "g2g_plan_wire": (_i, [_vp, _p32, _p32, _p32, _p32, _p32, _p32, _p64, _p64, _i64]),
     "g2g_plan_send_buffer": (_i, [_vp, _i, ctypes.POINTER(_vp), _p64]),
@@ -112,6 +118,8 @@ PROTOTYPES = {
                              _i64, _p64, _vp]),
     "g2g_optimal_reordering": (_i, [ctypes.POINTER(G2GEdge), _i64, ctypes.c_int32, _p32, _p32]),
     "g2g_fill_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, _vp]),
+    "g2g_scale_layout": (_i, [ctypes.POINTER(G2GLayout), _i, _vp, _vp]),
+    "g2g_copy_bytes": (_i, [_vp, _vp, _i64]),
     "g2g_stream_probe": (_i, [_i, _vp, _vp, _i64, _i, _vp, ctypes.POINTER(ctypes.c_double)]),
     "g2g_verify_layout": (_i, [ctypes.POINTER(G2GLayout), _i, ctypes.c_uint64, ctypes.c_char, _i64, _vp,
                                _p64]),

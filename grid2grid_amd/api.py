@@ -83,6 +83,12 @@ class Plan:
         b = _scalar_bytes(self.dtype, beta)
         check(_lib.load().g2g_transform(self.handle, a, b, _stream(stream)))
 
+    def transform_graph(self, stream, alpha=None, beta=None):
+        """The same launch replayed from the plan's hipGraph (g2g_transform_graph)."""
+        a = _scalar_bytes(self.dtype, alpha)
+        b = _scalar_bytes(self.dtype, beta)
+        check(_lib.load().g2g_transform_graph(self.handle, a, b, _stream(stream)))
+
     def transform_staged(self, alpha=None, beta=None, stream=None):
         a = _scalar_bytes(self.dtype, alpha)
         b = _scalar_bytes(self.dtype, beta)
@@ -242,6 +248,39 @@ def set_tuning(tile_bytes=0, launch_mode=-1):
     """Process-wide override of the work-item size (0 = automatic) and launch
     mode (-1 = automatic) for plans and launches made afterwards."""
     check(_lib.load().g2g_set_tuning(int(tile_bytes), int(launch_mode)))
+
+
+def transform_layouts(initial, init_base, final, final_base, dtype, trans="N", alpha=None, beta=None,
+                      stream=None):
+    """The drop-in call (g2g_transform_layouts): plans the layout pair on its
+    first call, replays the cached plan afterwards."""
+    es = DTYPE_SIZE.get(dtype, 8)
+    di = initial.descriptor(init_base, es)
+    df = final.descriptor(final_base, es)
+    a = _scalar_bytes(dtype, alpha)
+    b = _scalar_bytes(dtype, beta)
+    check(_lib.load().g2g_transform_layouts(di.ref(), df.ref(), dtype, trans.encode(), a, b, _stream(stream)))
+
+
+def plan_cache_stats():
+    """(hits, misses, entries) of the plan cache behind transform_layouts."""
+    h, m, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(_lib.load().g2g_plan_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(e)))
+    return h.value, m.value, e.value
+
+
+def plan_cache_clear():
+    check(_lib.load().g2g_plan_cache_clear())
+
+
+def plan_cache_set_capacity(entries):
+    check(_lib.load().g2g_plan_cache_set_capacity(int(entries)))
+
+
+def scale_layout(layout, base, dtype, beta, stream=None):
+    """grid_layout::scale_by on device memory (g2g_scale_layout)."""
+    d = layout.descriptor(base, DTYPE_SIZE[dtype])
+    check(_lib.load().g2g_scale_layout(d.ref(), dtype, _scalar_bytes(dtype, beta), _stream(stream)))
 
 
 def fill_layout(layout, base, dtype, seed, stream=None):

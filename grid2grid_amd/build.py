@@ -12,7 +12,13 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgrid2grid_amd.so")
 
-SOURCES = ["g2g_kernels.hip", "g2g_host.cpp", "g2g_serve.cpp", "g2g_rank_pairs.hip"]
+# (source, object name, defines): the gather's variants are compiled once per
+# (workgroup size, feature level) family so that they build in parallel
+SOURCES = [("g2g_kernels.hip", "g2g_kernels", []), ("g2g_host.cpp", "g2g_host", []),
+           ("g2g_serve.cpp", "g2g_serve", []), ("g2g_rank_pairs.hip", "g2g_rank_pairs", []),
+           ("g2g_cache.cpp", "g2g_cache", [])] + [
+    ("g2g_gather_variants.hip", f"g2g_gather_{th}_{feat}", [f"-DG2G_THREADS={th}", f"-DG2G_FEAT={feat}"])
+    for th in (256, 512) for feat in (0, 1, 2)]
 
 HIPCC_FLAGS = [
     "-std=c++17",
@@ -41,22 +47,51 @@ def hipcc():
     return "hipcc"
 
 
-def build_library(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + [
-        "-I" + os.path.join(ROOT, "include"),
-        "-I" + CSRC,
-    ] + srcs + ["-lpthread", "-o", tmp]
+def _compile(src, obj, defines, verbose):
+    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + defines + [
+        "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature; that warning is harmless
     noise = [l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l and l.strip()]
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(noise))
+        raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n" + "\n".join(noise))
     if verbose and noise:
         print("\n".join(noise), file=sys.stderr)
+
+
+def build_library(verbose=False, incremental=False):
+    """Every source compiled to an object (in parallel, under _build/), then
+    linked. incremental: keep objects newer than their source and every header
+    of csrc/ and include/ (for development; the default rebuilds all)."""
+    from concurrent.futures import ThreadPoolExecutor
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    headers += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                if f.endswith(".h")]
+    newest_header = max((os.path.getmtime(h) for h in headers), default=0)
+    jobs = []
+    objs = []
+    for name, oname, defines in SOURCES:
+        src = os.path.join(CSRC, name)
+        obj = os.path.join(obj_dir, oname + ".o")
+        objs.append(obj)
+        fresh = (incremental and os.path.exists(obj) and os.path.getmtime(obj) > os.path.getmtime(src)
+                 and os.path.getmtime(obj) > newest_header)
+        if not fresh:
+            jobs.append((src, obj, defines))
+    with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), 8))) as ex:
+        for f in [ex.submit(_compile, src, obj, defines, verbose) for src, obj, defines in jobs]:
+            f.result()
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-lpthread", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
     return LIB
 

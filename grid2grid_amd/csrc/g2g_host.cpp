@@ -91,6 +91,80 @@ struct dbuf {
     }
 };
 
+// ------------------------------------------------------ work counters
+//
+// A persistent launch pulls work indices from kQueueWords device counters,
+// reset by a kernel at the head of the launch. Launches on one stream run in
+// order, so every stream gets one set of counters of its own (created on the
+// stream's first persistent launch): launches of one plan or batch on
+// several streams never share a counter. A captured graph owns a set of its
+// own. Sets are carved from chunks that live until the process ends.
+constexpr int kCounterStride = 32;    // words per set (128 bytes: one set per cache line)
+constexpr int kCountersPerChunk = 256;
+
+struct counter_pool {
+    std::mutex mu;
+    std::vector<std::pair<std::pair<int, hipStream_t>, uint32_t *>> by_stream;
+    std::vector<std::pair<int, uint32_t *>> free_sets;  // (device, set)
+};
+counter_pool &counters() {
+    static counter_pool *pool = new counter_pool();  // never destroyed: kernels may still run at exit
+    return *pool;
+}
+
+// A fresh set on the current device (pool lock held).
+int counter_set_locked(counter_pool &pool, int dev, uint32_t **out) {
+    for (size_t k = 0; k < pool.free_sets.size(); ++k)
+        if (pool.free_sets[k].first == dev) {
+            *out = pool.free_sets[k].second;
+            pool.free_sets.erase(pool.free_sets.begin() + (std::ptrdiff_t)k);
+            return G2G_OK;
+        }
+    uint32_t *chunk = nullptr;
+    const size_t bytes = sizeof(uint32_t) * kCounterStride * kCountersPerChunk;
+    hipError_t e = hipMalloc(&chunk, bytes);
+    if (e != hipSuccess) return fail(G2G_ERR_NOMEM, "work counters: hipMalloc failed: %s", hipGetErrorString(e));
+    e = hipMemset(chunk, 0, bytes);
+    if (e != hipSuccess) return fail(G2G_ERR_HIP, "work counters: hipMemset failed: %s", hipGetErrorString(e));
+    for (int k = kCountersPerChunk - 1; k >= 1; --k) pool.free_sets.push_back({dev, chunk + k * kCounterStride});
+    *out = chunk;
+    return G2G_OK;
+}
+
+// The counters of launches on stream s.
+int stream_counters(hipStream_t s, uint32_t **out) {
+    int dev = 0;
+    G2G_HIP(hipGetDevice(&dev));
+    counter_pool &pool = counters();
+    std::lock_guard<std::mutex> g(pool.mu);
+    for (auto &e : pool.by_stream)
+        if (e.first.first == dev && e.first.second == s) {
+            *out = e.second;
+            return G2G_OK;
+        }
+    G2G_TRY(counter_set_locked(pool, dev, out));
+    pool.by_stream.push_back({{dev, s}, *out});
+    return G2G_OK;
+}
+
+// A set owned by one object (a captured graph); give it back with release_counters.
+int private_counters(uint32_t **out) {
+    int dev = 0;
+    G2G_HIP(hipGetDevice(&dev));
+    counter_pool &pool = counters();
+    std::lock_guard<std::mutex> g(pool.mu);
+    return counter_set_locked(pool, dev, out);
+}
+
+void release_counters(uint32_t *set) {
+    if (!set) return;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    counter_pool &pool = counters();
+    std::lock_guard<std::mutex> g(pool.mu);
+    pool.free_sets.push_back({dev, set});
+}
+
 // Tuning knobs (g2g_set_tuning).
 std::atomic<int64_t> g_tile_bytes{0};
 std::atomic<int> g_launch_mode{-1};
@@ -259,7 +333,26 @@ struct g2g_plan_s {
     uint64_t class_key = 0;          // shape class of the tuning table
     double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
     dbuf<uint32_t> ranks;            // (src << 16) | dst per piece
-    dbuf<uint32_t> queue;
+    // guards what is built on first use (wire order, staging, chunks) and
+    // the plan's captured graph
+    std::mutex mu;
+    // g2g_transform_graph: the launch captured for one (alpha, beta), its
+    // stream table and configuration; the graph owns its work counters
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t *gcounters = nullptr;
+    gather_args gargs;
+    launch_cfg gcfg;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+    ~g2g_plan_s() {
+        drop_graph();
+        release_counters(gcounters);
+    }
 
     // wire order (computed on demand)
     bool wire_ready = false;
@@ -279,6 +372,13 @@ namespace {
 // The table the fused gather runs over: the column-chunked stream when the
 // plan has one, else the pieces themselves.
 stage_table &gtab(g2g_plan p) { return p->use_chunk ? p->chunk : p->fused; }
+
+// The table a request runs over: scaled requests take the pieces in their own
+// order, whose rows and columns are the whole piece's (the scaled epilogue's
+// operand order depends on an element's position in the piece).
+stage_table &xtab(g2g_plan p, const kreqs &kr) {
+    return ((kr.r[0].flags | kr.r[1].flags) & REQ_SCALE) ? p->fused : gtab(p);
+}
 
 bool persistent_mode(uint32_t n_tiles) {
     const int p = g_launch_mode.load();
@@ -562,15 +662,27 @@ work_desc make_desc(const stage_table &t, const kreqs &kr, uint32_t tile_base) {
     return d;
 }
 
-int run_table(g2g_plan p, stage_table &t, const kreqs &kr, hipStream_t s) {
-    gather_args a;
+// The launch of one table: arguments and configuration. `queue`: the work
+// counters of a persistent launch (the stream's when null).
+int table_launch(const stage_table &t, const kreqs &kr, uint32_t *queue, hipStream_t s, gather_args &a,
+                 launch_cfg &c) {
     std::memset(&a, 0, sizeof a);
     a.one = make_desc(t, kr, 0);
     a.total_tiles = t.n_tiles;
-    a.queue = p->queue.p;
-    launch_cfg c = t.cfg;
+    c = t.cfg;
     if (g_launch_mode.load() >= 0) set_launch_mode(c, g_launch_mode.load());
     c.features = std::max(request_features(kr.r[0]), request_features(kr.r[1]));
+    if (c.persistent && t.n_tiles) {
+        if (!queue) G2G_TRY(stream_counters(s, &queue));
+        a.queue = queue;
+    }
+    return G2G_OK;
+}
+
+int run_table(const stage_table &t, const kreqs &kr, hipStream_t s, uint32_t *queue = nullptr) {
+    gather_args a;
+    launch_cfg c;
+    G2G_TRY(table_launch(t, kr, queue, s, a, c));
     G2G_HIP(launch_gather(a, c, device_cus(), s));
     return G2G_OK;
 }
@@ -722,7 +834,6 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     G2G_TRY(p->fused.pieces.alloc((size_t)np));
     G2G_TRY(p->fused.pend.alloc((size_t)np));
     G2G_TRY(p->ranks.alloc((size_t)np));
-    G2G_TRY(p->queue.alloc(kQueueWords));
     G2G_HIP(launch_plan_pieces(d_ur.p, (int32_t)n_ur, d_uc.p, (int32_t)n_uc, d_irs.p, d_ics.p, d_frs.p, d_fcs.p,
                                d_ic.p, wi.nbc, d_fc.p, wf.nbc, es, trans != 'N', p->fused.pieces.p,
                                p->fused.pend.p, p->ranks.p, s));
@@ -750,6 +861,12 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
         stage_table &gt = gtab(p);
         G2G_TRY(retile(gt, choose_tile_bytes(total), s));
         gt.cfg = default_cfg(gt.n_tiles);
+    }
+    // scaled requests run over the pieces themselves: tiled from the start,
+    // so that no launch has to build anything
+    if (p->use_chunk) {
+        G2G_TRY(retile(p->fused, choose_tile_bytes(total), s));
+        p->fused.cfg = default_cfg(p->fused.n_tiles);
     }
     *out = p;
     p = nullptr;  // released from the guard
@@ -782,6 +899,7 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
         t->launch_mode > G2G_LAUNCH_PERSISTENT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
     if (t->column_chunks < 0) return fail(G2G_ERR_INVALID, "column_chunks %d < 0", (int)t->column_chunks);
+    std::lock_guard<std::mutex> lock(p->mu);
     G2G_TRY(select_chunks(p, (uint32_t)t->column_chunks, nullptr));
     stage_table &gt = gtab(p);
     if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
@@ -807,6 +925,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     // configurations are then timed again on it and on column-chunked streams
     // of several widths (transposed plans: whole groups of 256 bytes of final
     // columns, the width of a transpose tile of 64 final rows)
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (p->use_chunk) ++p->stream_gen;  // batches built on the chunk stream must not run it
     p->use_chunk = false;
     const uint64_t total = p->fused.total;
     std::vector<uint64_t> sizes;
@@ -821,7 +941,7 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         std::memset(&a, 0, sizeof a);
         a.one = make_desc(gt, kr, 0);
         a.total_tiles = gt.n_tiles;
-        a.queue = p->queue.p;
+        if (c.persistent) G2G_TRY(stream_counters(s, &a.queue));
         hipError_t he = launch_gather(a, c, device_cus(), s);  // warm-up
         if (he == hipSuccess) he = hipEventRecord(e0, s);
         for (int r = 0; r < n && he == hipSuccess; ++r) he = launch_gather(a, c, device_cus(), s);
@@ -914,14 +1034,56 @@ int g2g_transform(g2g_plan p, const void *alpha, const void *beta, void *stream)
     std::memset(&kr, 0, sizeof kr);
     kr.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, true);
     kr.r[1] = kr.r[0];
-    return run_table(p, gtab(p), kr, (hipStream_t)stream);
+    return run_table(xtab(p, kr), kr, (hipStream_t)stream);
+}
+
+int g2g_transform_graph(g2g_plan p, const void *alpha, const void *beta, void *stream) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    if ((alpha == nullptr) != (beta == nullptr)) return fail(G2G_ERR_INVALID, "give both alpha and beta or neither");
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(G2G_ERR_INVALID, "graph replay needs a non-null stream");
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    kr.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, true);
+    kr.r[1] = kr.r[0];
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (!p->gcounters) G2G_TRY(private_counters(&p->gcounters));
+    gather_args a;
+    launch_cfg c;
+    G2G_TRY(table_launch(xtab(p, kr), kr, p->gcounters, s, a, c));
+    // the graph holds the launch's arguments: capture again when the request,
+    // the table or its configuration changed since
+    const bool same = p->gexec && std::memcmp(&a, &p->gargs, sizeof a) == 0 && c.threads == p->gcfg.threads &&
+                      c.unroll_bytes == p->gcfg.unroll_bytes && c.nontemporal == p->gcfg.nontemporal &&
+                      c.persistent == p->gcfg.persistent && c.xcd_ranges == p->gcfg.xcd_ranges &&
+                      c.features == p->gcfg.features;
+    if (!same) {
+        p->drop_graph();
+        G2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch_gather(a, c, device_cus(), s);
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &graph);
+        if (le != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(G2G_ERR_HIP, "graph capture: %s", hipGetErrorString(le != hipSuccess ? le : ee));
+        }
+        p->graph = graph;
+        G2G_HIP(hipGraphInstantiate(&p->gexec, graph, nullptr, nullptr, 0));
+        p->gargs = a;
+        p->gcfg = c;
+    }
+    G2G_HIP(hipGraphLaunch(p->gexec, s));
+    return G2G_OK;
 }
 
 int g2g_transform_staged(g2g_plan p, const void *alpha, const void *beta, void *stream) {
     if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
     if ((alpha == nullptr) != (beta == nullptr)) return fail(G2G_ERR_INVALID, "give both alpha and beta or neither");
     hipStream_t s = (hipStream_t)stream;
-    G2G_TRY(ensure_staged(p, s));
+    {
+        std::lock_guard<std::mutex> lock(p->mu);
+        G2G_TRY(ensure_staged(p, s));
+    }
     kreqs ka, kb, kc;
     std::memset(&ka, 0, sizeof ka);
     ka.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, false);  // pack
@@ -932,15 +1094,16 @@ int g2g_transform_staged(g2g_plan p, const void *alpha, const void *beta, void *
     kc = ka;
     kc.r[0] = make_request(p->dtype, 'N', alpha, beta, false, true);        // unpack
     kc.r[1] = kc.r[0];
-    G2G_TRY(run_table(p, p->sa, ka, s));
-    G2G_TRY(run_table(p, p->sb, kb, s));
-    G2G_TRY(run_table(p, p->sc, kc, s));
+    G2G_TRY(run_table(p->sa, ka, s));
+    G2G_TRY(run_table(p->sb, kb, s));
+    G2G_TRY(run_table(p->sc, kc, s));
     return G2G_OK;
 }
 
 int g2g_plan_counts(g2g_plan p, int64_t *counts, int64_t capacity) {
     if (!p || !counts) return fail(G2G_ERR_INVALID, "NULL argument");
     if (capacity < (int64_t)p->P * p->P) return fail(G2G_ERR_INVALID, "capacity below P*P");
+    std::lock_guard<std::mutex> lock(p->mu);
     G2G_TRY(ensure_wire(p, nullptr));
     for (size_t k = 0; k < p->counts.size(); ++k) counts[k] = (int64_t)p->counts[k];
     return G2G_OK;
@@ -952,6 +1115,7 @@ int g2g_plan_wire(g2g_plan p, int32_t *row0, int32_t *col0, int32_t *rows, int32
     if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
     const int64_t n = p->fused.n;
     if (capacity < n) return fail(G2G_ERR_INVALID, "capacity below n_pieces");
+    std::lock_guard<std::mutex> lock(p->mu);
     G2G_TRY(ensure_wire(p, nullptr));
     std::vector<uint32_t> rk((size_t)n);
     std::vector<uint64_t> w((size_t)n);
@@ -999,7 +1163,8 @@ struct g2g_batch_s {
     dbuf<work_desc> descs;
     dbuf<uint32_t> first;               // per plan: first piece of each batch work item
     dbuf<work_rec> work;  // per global work index: transform and first piece
-    dbuf<uint32_t> queue;
+    std::vector<const stage_table *> tabs;  // per plan: the table its request runs over
+    uint32_t *gcounters = nullptr;      // work counters of the captured graph
     uint32_t total_tiles = 0;
     uint64_t total_bytes = 0;
     uint64_t tile_bytes = 0;
@@ -1014,7 +1179,10 @@ struct g2g_batch_s {
         exec = nullptr;
         graph = nullptr;
     }
-    ~g2g_batch_s() { drop_graph(); }
+    ~g2g_batch_s() {
+        drop_graph();
+        release_counters(gcounters);
+    }
 };
 
 namespace {
@@ -1029,14 +1197,16 @@ int batch_check(g2g_batch b) {
     return G2G_OK;
 }
 
-int batch_launch(g2g_batch b, hipStream_t s) {
+// queue: the work counters (the stream's when null)
+int batch_launch(g2g_batch b, hipStream_t s, uint32_t *queue = nullptr) {
     G2G_TRY(batch_check(b));
     gather_args a;
     std::memset(&a, 0, sizeof a);
     a.descs = b->descs.p;
     a.work = b->work.p;
     a.total_tiles = b->total_tiles;
-    a.queue = b->queue.p;
+    if (!queue && b->total_tiles) G2G_TRY(stream_counters(s, &queue));
+    a.queue = queue;
     launch_cfg c = b->cfg;
     c.features = FEAT_COPY;
     for (const work_desc &d : b->host_descs)
@@ -1055,14 +1225,14 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
     uint64_t tiles = 0;
     for (size_t k = 0; k < b->plans.size(); ++k) {
         off[k] = tiles;
-        tiles += (gtab(b->plans[k]).total + tile_bytes - 1) / tile_bytes;
+        tiles += (b->tabs[k]->total + tile_bytes - 1) / tile_bytes;
     }
     if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
     b->total_tiles = (uint32_t)tiles;
     G2G_TRY(b->first.alloc(tiles));
     G2G_TRY(b->work.alloc(tiles));
     for (size_t k = 0; k < b->plans.size(); ++k) {
-        const stage_table &t = gtab(b->plans[k]);
+        const stage_table &t = *b->tabs[k];
         work_desc &d = b->host_descs[k];
         d.first_piece = b->first.p + off[k];
         d.tile_bytes = tile_bytes;
@@ -1093,9 +1263,10 @@ int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas, c
         std::memset(&kr, 0, sizeof kr);
         kr.r[0] = make_request(plans[k]->dtype, plans[k]->trans, a, be, true, true);
         kr.r[1] = kr.r[0];
-        out.push_back(make_desc(gtab(plans[k]), kr, (uint32_t)tiles));
-        tiles += gtab(plans[k]).n_tiles;
-        bytes += gtab(plans[k]).total;
+        const stage_table &t = xtab(plans[k], kr);
+        out.push_back(make_desc(t, kr, (uint32_t)tiles));
+        tiles += t.n_tiles;
+        bytes += t.total;
         if (tiles > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items in one launch");
     }
     total_tiles = (uint32_t)tiles;
@@ -1154,8 +1325,10 @@ int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
     uint32_t tiles = 0;
     G2G_TRY(internal_descs(n, plans, alphas, betas, b->host_descs, tiles, b->total_bytes));
     b->plans.assign(plans, plans + n);
-    for (int k = 0; k < n; ++k) b->plan_gen.push_back(plans[k]->stream_gen);
-    G2G_TRY(b->queue.alloc(kQueueWords));
+    for (int k = 0; k < n; ++k) {
+        b->plan_gen.push_back(plans[k]->stream_gen);
+        b->tabs.push_back(&xtab(plans[k], b->host_descs[k].kr));
+    }
     // batches always run persistent; g2g_set_tuning may ask for per-XCD ranges
     set_launch_mode(b->cfg, g_launch_mode.load() == G2G_LAUNCH_PERSISTENT_XCD ? G2G_LAUNCH_PERSISTENT_XCD
                                                                            : G2G_LAUNCH_PERSISTENT);
@@ -1177,8 +1350,9 @@ int g2g_batch_execute_graph(g2g_batch b, void *stream) {
     if (!s) return fail(G2G_ERR_INVALID, "graph replay needs a non-null stream");
     G2G_TRY(batch_check(b));
     if (!b->exec) {
+        if (!b->gcounters) G2G_TRY(private_counters(&b->gcounters));
         G2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = batch_launch(b, s);
+        int rc = batch_launch(b, s, b->gcounters);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (rc != G2G_OK) {
@@ -1311,6 +1485,27 @@ int g2g_fill_layout(const g2g_layout *layout, int dtype, uint64_t seed, void *st
     G2G_TRY(d.upload(fb.data(), fb.size(), s));
     G2G_HIP(launch_fill(d.p, (int32_t)fb.size(), dtype, seed, n_cols, s));
     G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+int g2g_scale_layout(const g2g_layout *layout, int dtype, const void *beta, void *stream) {
+    if (!dtype_size(dtype) || !beta) return fail(G2G_ERR_INVALID, "bad arguments");
+    std::vector<fill_block> fb;
+    int64_t n_cols = 0;
+    G2G_TRY(fill_table(layout, fb, n_cols));
+    const request rq = make_request(dtype, 'N', beta, beta, false, false);
+    if (is_one(dtype, beta)) return G2G_OK;  // grid_layout.hpp:50, block.cpp:293: beta == 1 is a no-op
+    hipStream_t s = (hipStream_t)stream;
+    dbuf<fill_block> d;
+    G2G_TRY(d.upload(fb.data(), fb.size(), s));
+    G2G_HIP(launch_scale(d.p, (int32_t)fb.size(), rq, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
+int g2g_copy_bytes(void *dst, const void *src, int64_t bytes) {
+    if (bytes < 0 || (bytes && (!dst || !src))) return fail(G2G_ERR_INVALID, "bad arguments");
+    if (bytes) G2G_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDefault));
     return G2G_OK;
 }
 

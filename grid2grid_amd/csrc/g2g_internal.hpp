@@ -21,6 +21,20 @@ enum : uint32_t {
     REQ_SCALE = 4u,  // dst = beta*dst + alpha*op(src)
 };
 
+// piece.req: bit 0 selects the launch's second request; bits 1-2 name the
+// reference call site that scales the piece (only read when the request
+// scales): the per-column loop of copy2D_and_scale, its one loop over a
+// contiguous piece, or copy_transpose_and_scale of a local transposed piece
+// (reference memory_utils.hpp:79-122, 186-256).
+enum : uint32_t {
+    PIECE_REQ1 = 1u,
+    PIECE_SITE_SHIFT = 1u,
+    PIECE_SITE_MASK = 3u << 1,
+    PIECE_SITE_COL = 0u,
+    PIECE_SITE_FLAT = 1u,
+    PIECE_SITE_TRANS = 2u
+};
+
 // Per-request parameters: element type, operation and scalars (raw bits).
 struct request {
     uint32_t dtype;
@@ -236,6 +250,8 @@ struct fill_block {
 };
 hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
                        int64_t n_cols, hipStream_t stream);
+// every element of every block times beta (rq.beta, rq.dtype, rq.esize)
+hipError_t launch_scale(const fill_block *blocks, int32_t n_blocks, const request &rq, hipStream_t stream);
 hipError_t launch_verify(const fill_block *blocks, int32_t n_blocks, int dtype, uint64_t seed,
                          int trans, int conj, int64_t a_n_cols, unsigned long long *bad,
                          hipStream_t stream);

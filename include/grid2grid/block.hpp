@@ -1,9 +1,11 @@
 // grid2grid::block<T>, local_blocks<T>: a column-major block of device memory
 // and the blocks of a layout. Names, constructors and flags follow the
 // reference (src/grid2grid/block.hpp:23-190, block.cpp:23-348); `data` is a
-// device pointer here, so element accessors that dereference it on the host
-// are not provided.
+// device pointer here, so local_element returns a proxy that reads and
+// writes the element in device memory, and scale_by runs on the device.
 #pragma once
+
+#include <grid2grid_amd.h>
 
 #include <grid2grid/grid2D.hpp>
 #include <grid2grid/interval.hpp>
@@ -14,10 +16,48 @@
 #include <cstddef>
 #include <ostream>
 #include <stdexcept>
+#include <string>
 #include <utility>
 #include <vector>
 
 namespace grid2grid {
+
+namespace detail {
+
+template <typename T> struct dtype_of;
+template <> struct dtype_of<float> { static constexpr int value = G2G_FLOAT32; };
+template <> struct dtype_of<double> { static constexpr int value = G2G_FLOAT64; };
+template <> struct dtype_of<std::complex<float>> { static constexpr int value = G2G_COMPLEX64; };
+template <> struct dtype_of<std::complex<double>> { static constexpr int value = G2G_COMPLEX128; };
+
+inline void check(int rc) {
+    if (rc != G2G_OK) throw std::runtime_error(std::string("grid2grid: ") + g2g_last_error());
+}
+
+}  // namespace detail
+
+// One element of device memory: converts to T by reading it and takes a T by
+// writing it (each a synchronous copy), as block::local_element's reference
+// does on host memory.
+template <typename T>
+class device_ref {
+  public:
+    explicit device_ref(T *p) : p_(p) {}
+    operator T() const {
+        T v;
+        detail::check(g2g_copy_bytes(&v, p_, sizeof(T)));
+        return v;
+    }
+    device_ref &operator=(const T &v) {
+        detail::check(g2g_copy_bytes(p_, &v, sizeof(T)));
+        return *this;
+    }
+    device_ref &operator=(const device_ref &o) { return *this = T(o); }
+    T *address() const { return p_; }
+
+  private:
+    T *p_;
+};
 
 struct block_coordinates {
     int row = 0;
@@ -123,6 +163,29 @@ struct block {
     int n_cols() const { return cols_interval.length(); }
     std::pair<int, int> size() const { return {n_rows(), n_cols()}; }
     size_t total_size() const { return size_t(n_rows()) * size_t(n_cols()); }
+
+    // Element (li, lj) of the block, local coordinates, swapped first for a
+    // transposed block (block.cpp:228-247): its device address, and a proxy
+    // that reads or writes it.
+    T *local_element_address(int li, int lj) const {
+        if (transpose_on_copy) std::swap(li, lj);
+        return data + std::ptrdiff_t(stride) * lj + li;
+    }
+    device_ref<T> local_element(int li, int lj) const { return device_ref<T>(local_element_address(li, lj)); }
+
+    // Every element times beta on the device (block.cpp:291-306); nothing
+    // when beta == 1. A block flagged for transposition is refused (the
+    // reference asserts it is not).
+    void scale_by(T beta, void *stream = nullptr) {
+        if (beta == T{1}) return;
+        if (transpose_on_copy) throw std::runtime_error("grid2grid: scale_by of a block flagged for transposition");
+        const int32_t rs[2] = {rows_interval.start, rows_interval.end};
+        const int32_t cs[2] = {cols_interval.start, cols_interval.end};
+        const int32_t owner = 0;
+        const g2g_block b{0, 0, stride, (void *)data};
+        const g2g_layout l{2, 2, rs, cs, &owner, 1, 1, &b};
+        detail::check(g2g_scale_layout(&l, detail::dtype_of<T>::value, &beta, stream));
+    }
 
     // flags only: the data moves during the transform
     void transpose_or_conjugate(char flag) {

@@ -43,6 +43,31 @@ class grid_layout {
     int num_blocks_col() const noexcept { return grid.num_blocks_col(); }
     int num_blocks_row() const noexcept { return grid.num_blocks_row(); }
 
+    // Every element of every block times beta on the device, one launch for
+    // the whole layout (grid_layout.hpp:48-56); nothing when beta == 1.
+    // Blocks flagged for transposition are refused (block.cpp:294).
+    void scale_by(const T beta, void *stream = nullptr) {
+        if (beta == T{1}) return;
+        const grid2D &g = grid.grid();
+        std::vector<int32_t> rs(g.rows_split.begin(), g.rows_split.end());
+        std::vector<int32_t> cs(g.cols_split.begin(), g.cols_split.end());
+        std::vector<int32_t> owners(size_t(g.n_rows) * g.n_cols, 0);
+        std::vector<g2g_block> bl;
+        for (int k = 0; k < blocks.num_blocks(); ++k) {
+            const block<T> &b = blocks.get_block(k);
+            if (b.transpose_on_copy)
+                throw std::runtime_error("grid2grid: scale_by of a block flagged for transposition");
+            const int r = b.coordinates.row, c = b.coordinates.col;
+            if (r < 0 || r >= g.n_rows || c < 0 || c >= g.n_cols || b.rows_interval != g.row_interval(r) ||
+                b.cols_interval != g.col_interval(c))
+                throw std::runtime_error("grid2grid: a block does not match its grid cell");
+            bl.push_back(g2g_block{r, c, b.stride, (void *)b.data});
+        }
+        const g2g_layout l{int32_t(rs.size()), int32_t(cs.size()), rs.data(), cs.data(), owners.data(), 1,
+                           int32_t(bl.size()), bl.data()};
+        detail::check(g2g_scale_layout(&l, detail::dtype_of<T>::value, &beta, stream));
+    }
+
     assigned_grid2D grid;
     local_blocks<T> blocks;
 };

@@ -4,10 +4,12 @@
 // layouts are virtual ranks whose blocks all live in this GPU's memory.
 //
 // As in the reference, transform() returns when the final layout holds the
-// data (unless virtual_comm::synchronize is false), the overloads with
+// data (unless virtual_comm::synchronize is false) and the overloads with
 // `trans` leave the initial layout transposed/conjugated (transform.cpp:151,
-// 197), and nothing is cached between calls; transform_plan keeps the plan
-// (grid cover, pieces, offsets) of a layout pair for repeated use.
+// 197). Unlike the reference, which plans every call, the fused single-pair
+// overloads plan a layout pair once: g2g_transform_layouts keeps the plan
+// (grid cover, pieces, offsets) in a process-wide cache keyed by the layouts'
+// descriptors. transform_plan holds the plan of a layout pair explicitly.
 #pragma once
 
 #include <grid2grid_amd.h>
@@ -36,16 +38,6 @@ struct virtual_comm {
 };
 
 namespace detail {
-
-template <typename T> struct dtype_of;
-template <> struct dtype_of<float> { static constexpr int value = G2G_FLOAT32; };
-template <> struct dtype_of<double> { static constexpr int value = G2G_FLOAT64; };
-template <> struct dtype_of<std::complex<float>> { static constexpr int value = G2G_COMPLEX64; };
-template <> struct dtype_of<std::complex<double>> { static constexpr int value = G2G_COMPLEX128; };
-
-inline void check(int rc) {
-    if (rc != G2G_OK) throw std::runtime_error(std::string("grid2grid: ") + g2g_last_error());
-}
 
 // g2g_layout view of a grid_layout (storage orientation required).
 template <typename T>
@@ -108,6 +100,14 @@ class transform_plan {
                                                         : g2g_transform_staged(p_, a, b, comm.stream));
         if (comm.synchronize) detail::check(g2g_synchronize(comm.stream));
     }
+    // the same launch replayed from the plan's hipGraph (g2g_transform_graph);
+    // comm.stream must not be the null stream
+    void execute_graph(T alpha = T{1}, T beta = T{0}, const virtual_comm &comm = {}) {
+        const bool plain = alpha == T{1} && beta == T{0};
+        const void *a = plain ? nullptr : &alpha, *b = plain ? nullptr : &beta;
+        detail::check(g2g_transform_graph(p_, a, b, comm.stream));
+        if (comm.synchronize) detail::check(g2g_synchronize(comm.stream));
+    }
     g2g_plan_info info() const {
         g2g_plan_info i;
         detail::check(g2g_plan_get_info(p_, &i));
@@ -127,11 +127,31 @@ class transform_plan {
     g2g_plan p_ = nullptr;
 };
 
+namespace detail {
+
+// One pair through the plan cache (fused form) or a plan of its own (staged form).
+template <typename T>
+void transform_pair(grid_layout<T> &initial_layout, grid_layout<T> &final_layout, char trans, const T &alpha,
+                    const T &beta, const virtual_comm &comm) {
+    trans = upper(trans);
+    const bool plain = alpha == T{1} && beta == T{0};
+    if (comm.mode == exchange_mode::staged) {
+        transform_plan<T> p(initial_layout, final_layout, trans, comm.stream);
+        p.execute(alpha, beta, comm);
+        return;
+    }
+    layout_desc<T> di(initial_layout), df(final_layout);
+    check(g2g_transform_layouts(&di.l, &df.l, dtype_of<T>::value, trans, plain ? nullptr : &alpha,
+                                plain ? nullptr : &beta, comm.stream));
+    if (comm.synchronize) check(g2g_synchronize(comm.stream));
+}
+
+}  // namespace detail
+
 // final_layout = initial_layout
 template <typename T>
 void transform(grid_layout<T> &initial_layout, grid_layout<T> &final_layout, const virtual_comm &comm = {}) {
-    transform_plan<T> p(initial_layout, final_layout, 'N', comm.stream);
-    p.execute(T{1}, T{0}, comm);
+    detail::transform_pair<T>(initial_layout, final_layout, 'N', T{1}, T{0}, comm);
 }
 
 // final_layout = beta * final_layout + alpha * op(initial_layout); leaves
@@ -139,10 +159,7 @@ void transform(grid_layout<T> &initial_layout, grid_layout<T> &final_layout, con
 template <typename T>
 void transform(grid_layout<T> &initial_layout, grid_layout<T> &final_layout, const char trans, const T alpha,
                const T beta, const virtual_comm &comm = {}) {
-    {
-        transform_plan<T> p(initial_layout, final_layout, trans, comm.stream);
-        p.execute(alpha, beta, comm);
-    }
+    detail::transform_pair<T>(initial_layout, final_layout, trans, alpha, beta, comm);
     initial_layout.transpose_or_conjugate(trans);
 }
 

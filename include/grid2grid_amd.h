@@ -133,11 +133,44 @@ int g2g_plan_get_info(g2g_plan plan, g2g_plan_info *info);
  * alpha/beta point to one host element of the plan's dtype (NULL, NULL =
  * 1 and 0 = plain copy). Every element of every final block is written;
  * bytes outside the pieces (ld padding, gaps) are never touched.
- * Asynchronous on `stream`. A plan holds one work counter and one set of
- * staging buffers, so launches of the same plan (fused or staged) must not
- * overlap: issue them on one stream, or order them with events; plans built
- * for the same layouts can run side by side on different streams. */
+ * Asynchronous on `stream`. The work counters of a launch belong to its
+ * stream (each stream gets its own on its first launch, which allocates
+ * them once), so one plan may run on several streams at once and be used
+ * from several host threads; only the plain form (g2g_transform_staged) and
+ * the retuning calls (g2g_plan_set_tuning, g2g_plan_autotune) need launches
+ * of the same plan to be ordered. The result is the same whatever the
+ * launch configuration, including scaled non-finite values: the scaled
+ * epilogue reproduces the reference build's x86 arithmetic (NaN operand
+ * choice, default NaN, infinities recovered by the complex product). */
 int g2g_transform(g2g_plan plan, const void *alpha, const void *beta, void *stream);
+
+/* The same launch replayed from a hipGraph held by the plan: captured on
+ * first use and again whenever alpha/beta or the plan's configuration
+ * changed. The graph owns its work counters. `stream` must not be the null
+ * stream. Replays of one plan's graph must not overlap; the replay runs
+ * alongside g2g_transform launches of the same plan on other streams. */
+int g2g_transform_graph(g2g_plan plan, const void *alpha, const void *beta, void *stream);
+
+/* The drop-in call: grid2grid::transform(initial, final, trans, alpha, beta,
+ * comm) in one call (reference transform.hpp:13-43, transform.cpp:126-164),
+ * planned once per layout pair. The plan of (initial, final, dtype, trans) --
+ * splits, owners, rank counts, block coordinates, strides and addresses as
+ * given -- is kept in a process-wide cache (least recently used first out,
+ * 64 entries by default), so the reference's per-call planning
+ * (utils.hpp:146-192) happens on the first call only; the call then runs
+ * g2g_transform on `stream`. Safe from several host threads. A plan leaving
+ * the cache waits for the device to finish before its tables are freed. */
+int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final, int dtype, char trans,
+                          const void *alpha, const void *beta, void *stream);
+
+/* Capacity of the plan cache (entries; 0 disables keeping plans). */
+int g2g_plan_cache_set_capacity(int64_t entries);
+
+/* Drop every cached plan and reset the counters. */
+int g2g_plan_cache_clear(void);
+
+/* Calls served from the cache, calls that planned, plans held. Any pointer may be NULL. */
+int g2g_plan_cache_stats(int64_t *hits, int64_t *misses, int64_t *entries);
 
 /* The plain form, mirroring the reference step for step: pack every remote
  * piece into the per-rank send buffers in the reference's wire order
@@ -177,19 +210,23 @@ typedef struct g2g_batch_s *g2g_batch;
  * and transformer<T> (transform.cpp:166-204, transformer.hpp:8-62), without
  * their single-T restriction. alphas[k]/betas[k] point to one element of plan
  * k's dtype, or the arrays are NULL (plain copies). Plans must outlive the
- * batch. The batch runs each plan's stream as tuned at creation: after a plan
- * is retuned (g2g_plan_set_tuning, g2g_plan_autotune) the batch's launches
- * fail with G2G_ERR_INVALID until it is built again. */
+ * batch. The batch runs each plan's piece stream as it was at creation:
+ * once a plan's stream changes (column chunks switched on or off or to
+ * another width, by g2g_plan_set_tuning or g2g_plan_autotune) the batch's
+ * launches fail with G2G_ERR_INVALID until it is built again; a new work-item
+ * size or kernel variant of the plan leaves the batch as it is. */
 int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
                      const void *const *betas, void *stream, g2g_batch *out);
 
 /* Run every transform of the batch in one persistent kernel that pulls
- * (transform, tile) work items from a device queue. Launches of one batch
- * must not overlap (one work counter per batch). */
+ * (transform, tile) work items from a device queue (the stream's work
+ * counters, as for g2g_transform: launches of one batch on several streams
+ * may overlap). */
 int g2g_batch_execute(g2g_batch batch, void *stream);
 
 /* Same work replayed from a hipGraph captured on first use (the queue reset
- * is a kernel node, not a memset). */
+ * is a kernel node, not a memset); the graph owns its work counters, so its
+ * replays must not overlap each other. */
 int g2g_batch_execute_graph(g2g_batch batch, void *stream);
 
 /* Autotuner of a batch (declared with g2g_tuning below): times the batch's own
@@ -340,6 +377,24 @@ int g2g_comm_volume(const g2g_layout *initial, const g2g_layout *final, g2g_edge
  * rank is mapped to another. */
 int g2g_optimal_reordering(const g2g_edge *edges, int64_t n_edges, int32_t n_ranks,
                            int32_t *permutation, int32_t *reordered);
+
+/* ------------------------------------------------------ layout utilities */
+
+/* Replaces grid_layout<T>::scale_by and block<T>::scale_by (reference
+ * grid_layout.hpp:48-56, block.hpp:136, block.cpp:291-306): every element of
+ * every block of `layout` (its rows and columns by the splits, at the block's
+ * stride) is multiplied by *beta, one element of `dtype`; nothing happens
+ * when *beta == 1. The products are the reference build's (data[offset] *=
+ * beta with its x86 arithmetic, NaN payloads and complex infinities
+ * included). Padding and gaps are not touched. Synchronous on `stream`
+ * (the block table is uploaded for the call). */
+int g2g_scale_layout(const g2g_layout *layout, int dtype, const void *beta, void *stream);
+
+/* Synchronous copy of `bytes` between any host or device addresses
+ * (hipMemcpy, direction inferred). The facade's block<T>::local_element
+ * (reference block.hpp:130-131) reads and writes one element of device
+ * memory through it. */
+int g2g_copy_bytes(void *dst, const void *src, int64_t bytes);
 
 /* ------------------------------------------------ test-support kernels */
 

@@ -22,6 +22,7 @@
  */
 #include "g2g_oracle.h"
 
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -317,58 +318,309 @@ static void conj_el(int dtype, unsigned char *el) {
     if (dtype == ORC_C128) el[15] ^= 0x80;
 }
 
-/* dst = beta*dst + alpha*src for one element (memory_utils.hpp:37). */
+/* ----------------------------------------------- the scaled epilogue
+ *
+ * dst = beta*dst + alpha*src (memory_utils.hpp:37, 237, 251), restated on bit
+ * patterns with the arithmetic of the reference's x86-64 build (SSE2, no
+ * FMA, complex products as GCC lowers std::complex operator*):
+ *   - a product, sum or difference of two numbers is the IEEE round-to-nearest
+ *     result; an invalid operation (inf*0, inf-inf) gives the x86 default NaN
+ *     (sign set, quiet bit set, zero payload);
+ *   - an operation with a NaN operand returns the FIRST source operand if it
+ *     is a NaN, else the other one, with its quiet bit set (Intel SDM vol. 1,
+ *     table 4-7, SSE column). Which operand is the first source is a property
+ *     of the compiled reference, per call site and position; ref_order()
+ *     below states it;
+ *   - a complex product is computed inline as (ac - bd, ad + bc); when both
+ *     parts are NaN it is recomputed by libgcc's __mulsc3/__muldc3, the
+ *     algorithm of C11 Annex G.5.1 (example _Cmultd), including its recovery
+ *     of infinities.
+ * Operand order never changes a result without NaN operands. */
+
+enum {
+    ORD_M1S = 1,  /* beta*dst: beta is the first source (complex: first __mul?c3 argument) */
+    ORD_M2S = 2,  /* alpha*src: alpha is the first source (complex: first __mul?c3 argument) */
+    ORD_ARE = 4,  /* real part of the sum: alpha*src is the first source */
+    ORD_AIM = 8   /* imaginary part of the sum: alpha*src is the first source */
+};
+
+static int nan32(uint32_t x) { return (x & 0x7fffffffu) > 0x7f800000u; }
+static int nan64(uint64_t x) { return (x & 0x7fffffffffffffffull) > 0x7ff0000000000000ull; }
+static int inf32(uint32_t x) { return (x & 0x7fffffffu) == 0x7f800000u; }
+static int inf64(uint64_t x) { return (x & 0x7fffffffffffffffull) == 0x7ff0000000000000ull; }
+static float f32_of(uint32_t x) { float f; memcpy(&f, &x, 4); return f; }
+static uint32_t bits32(float f) { uint32_t x; memcpy(&x, &f, 4); return x; }
+static double f64_of(uint64_t x) { double f; memcpy(&f, &x, 8); return f; }
+static uint64_t bits64(double f) { uint64_t x; memcpy(&x, &f, 8); return x; }
+
+#define NAN_RULE32(a, b, expr)                                  \
+    do {                                                        \
+        if (nan32(a)) return (a) | 0x00400000u;                 \
+        if (nan32(b)) return (b) | 0x00400000u;                 \
+        uint32_t r_ = bits32(expr);                             \
+        return nan32(r_) ? 0xffc00000u : r_;                    \
+    } while (0)
+#define NAN_RULE64(a, b, expr)                                  \
+    do {                                                        \
+        if (nan64(a)) return (a) | 0x0008000000000000ull;       \
+        if (nan64(b)) return (b) | 0x0008000000000000ull;       \
+        uint64_t r_ = bits64(expr);                             \
+        return nan64(r_) ? 0xfff8000000000000ull : r_;          \
+    } while (0)
+
+/* a is the first source operand */
+static uint32_t mul32(uint32_t a, uint32_t b) { NAN_RULE32(a, b, f32_of(a) * f32_of(b)); }
+static uint32_t add32(uint32_t a, uint32_t b) { NAN_RULE32(a, b, f32_of(a) + f32_of(b)); }
+static uint32_t sub32(uint32_t a, uint32_t b) { NAN_RULE32(a, b, f32_of(a) - f32_of(b)); }
+static uint64_t mul64(uint64_t a, uint64_t b) { NAN_RULE64(a, b, f64_of(a) * f64_of(b)); }
+static uint64_t add64(uint64_t a, uint64_t b) { NAN_RULE64(a, b, f64_of(a) + f64_of(b)); }
+static uint64_t sub64(uint64_t a, uint64_t b) { NAN_RULE64(a, b, f64_of(a) - f64_of(b)); }
+
+/* copysign(isinf(x) ? 1 : 0, x) and copysign(0, x) on bit patterns */
+static uint32_t box32(uint32_t x) { return (x & 0x80000000u) | (inf32(x) ? 0x3f800000u : 0u); }
+static uint64_t box64(uint64_t x) {
+    return (x & 0x8000000000000000ull) | (inf64(x) ? 0x3ff0000000000000ull : 0ull);
+}
+
+/* libgcc __mulsc3(a, b, c, d) = (a + bi)(c + di): C11 Annex G.5.1 _Cmultd.
+ * Its products are a*c, b*d, a*d and c*b (first source first). */
+static void mulsc3(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t *re, uint32_t *im) {
+    uint32_t ac = mul32(a, c), bd = mul32(b, d), ad = mul32(a, d), bc = mul32(c, b);
+    uint32_t x = sub32(ac, bd), y = add32(ad, bc);
+    if (nan32(x) && nan32(y)) {
+        int recalc = 0;
+        if (inf32(a) || inf32(b)) {  /* the first factor is infinite: box it, NaNs of the other -> 0 */
+            a = box32(a);
+            b = box32(b);
+            if (nan32(c)) c &= 0x80000000u;
+            if (nan32(d)) d &= 0x80000000u;
+            recalc = 1;
+        }
+        if (inf32(c) || inf32(d)) {
+            c = box32(c);
+            d = box32(d);
+            if (nan32(a)) a &= 0x80000000u;
+            if (nan32(b)) b &= 0x80000000u;
+            recalc = 1;
+        }
+        if (!recalc && (inf32(ac) || inf32(bd) || inf32(ad) || inf32(bc))) {  /* overflow */
+            if (nan32(a)) a &= 0x80000000u;
+            if (nan32(b)) b &= 0x80000000u;
+            if (nan32(c)) c &= 0x80000000u;
+            if (nan32(d)) d &= 0x80000000u;
+            recalc = 1;
+        }
+        if (recalc) {
+            x = mul32(0x7f800000u, sub32(mul32(a, c), mul32(b, d)));
+            y = mul32(0x7f800000u, add32(mul32(a, d), mul32(b, c)));
+        }
+    }
+    *re = x;
+    *im = y;
+}
+
+static void muldc3(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint64_t *re, uint64_t *im) {
+    uint64_t ac = mul64(a, c), bd = mul64(b, d), ad = mul64(a, d), bc = mul64(c, b);
+    uint64_t x = sub64(ac, bd), y = add64(ad, bc);
+    const uint64_t S = 0x8000000000000000ull;
+    if (nan64(x) && nan64(y)) {
+        int recalc = 0;
+        if (inf64(a) || inf64(b)) {
+            a = box64(a);
+            b = box64(b);
+            if (nan64(c)) c &= S;
+            if (nan64(d)) d &= S;
+            recalc = 1;
+        }
+        if (inf64(c) || inf64(d)) {
+            c = box64(c);
+            d = box64(d);
+            if (nan64(a)) a &= S;
+            if (nan64(b)) b &= S;
+            recalc = 1;
+        }
+        if (!recalc && (inf64(ac) || inf64(bd) || inf64(ad) || inf64(bc))) {
+            if (nan64(a)) a &= S;
+            if (nan64(b)) b &= S;
+            if (nan64(c)) c &= S;
+            if (nan64(d)) d &= S;
+            recalc = 1;
+        }
+        if (recalc) {
+            x = mul64(0x7ff0000000000000ull, sub64(mul64(a, c), mul64(b, d)));
+            y = mul64(0x7ff0000000000000ull, add64(mul64(a, d), mul64(b, c)));
+        }
+    }
+    *re = x;
+    *im = y;
+}
+
+/* std::complex product l*r as GCC emits it: inline (ac - bd, ad + bc), and
+ * __mul?c3 with the arguments in the order the call site passes them when
+ * both parts are NaN (swap: r's parts first). Without NaN operands the inline
+ * order of the four products does not matter, and with one both parts are NaN. */
+static void cmul32(const uint32_t *l, const uint32_t *r, int swap, uint32_t *out) {
+    uint32_t x = sub32(mul32(l[0], r[0]), mul32(l[1], r[1]));
+    uint32_t y = add32(mul32(l[0], r[1]), mul32(l[1], r[0]));
+    if (nan32(x) && nan32(y)) {
+        if (swap)
+            mulsc3(r[0], r[1], l[0], l[1], &x, &y);
+        else
+            mulsc3(l[0], l[1], r[0], r[1], &x, &y);
+    }
+    out[0] = x;
+    out[1] = y;
+}
+
+static void cmul64(const uint64_t *l, const uint64_t *r, int swap, uint64_t *out) {
+    uint64_t x = sub64(mul64(l[0], r[0]), mul64(l[1], r[1]));
+    uint64_t y = add64(mul64(l[0], r[1]), mul64(l[1], r[0]));
+    if (nan64(x) && nan64(y)) {
+        if (swap)
+            muldc3(r[0], r[1], l[0], l[1], &x, &y);
+        else
+            muldc3(l[0], l[1], r[0], r[1], &x, &y);
+    }
+    out[0] = x;
+    out[1] = y;
+}
+
+/* The restated complex product l*r (swap: __mul?c3 takes r's parts first),
+ * and libgcc's own __mulsc3 / __muldc3 on the same parts, for tests. */
+void orc_cmul(int dtype, const void *l, const void *r, int swap, void *out) {
+    if (dtype == ORC_C64) {
+        uint32_t a[2], b[2], o[2];
+        memcpy(a, l, 8);
+        memcpy(b, r, 8);
+        cmul32(a, b, swap, o);
+        memcpy(out, o, 8);
+    } else {
+        uint64_t a[2], b[2], o[2];
+        memcpy(a, l, 16);
+        memcpy(b, r, 16);
+        cmul64(a, b, swap, o);
+        memcpy(out, o, 16);
+    }
+}
+
+extern float _Complex __mulsc3(float, float, float, float);
+extern double _Complex __muldc3(double, double, double, double);
+void orc_libgcc_mulxc3(int dtype, const void *l, const void *r, void *out) {
+    if (dtype == ORC_C64) {
+        float a[2], b[2];
+        memcpy(a, l, 8);
+        memcpy(b, r, 8);
+        float _Complex z = __mulsc3(a[0], a[1], b[0], b[1]);
+        memcpy(out, &z, 8);
+    } else {
+        double a[2], b[2];
+        memcpy(a, l, 16);
+        memcpy(b, r, 16);
+        double _Complex z = __muldc3(a[0], a[1], b[0], b[1]);
+        memcpy(out, &z, 16);
+    }
+}
+
+/* dst = beta*dst + alpha*src for one element, operand order `ord` (ORD_*). */
 static void scale_el(int dtype, unsigned char *dst, const unsigned char *src, const void *alpha,
-                     const void *beta) {
+                     const void *beta, int ord) {
     switch (dtype) {
     case ORC_F32: {
-        float a = *(const float *)alpha, b = *(const float *)beta, s, d;
+        uint32_t a, b, s, d;
+        memcpy(&a, alpha, 4);
+        memcpy(&b, beta, 4);
         memcpy(&s, src, 4);
         memcpy(&d, dst, 4);
-        float bd = b * d;
-        float as = a * s;
-        float r = bd + as;
+        uint32_t bd = (ord & ORD_M1S) ? mul32(b, d) : mul32(d, b);
+        uint32_t as = (ord & ORD_M2S) ? mul32(a, s) : mul32(s, a);
+        uint32_t r = (ord & ORD_ARE) ? add32(as, bd) : add32(bd, as);
         memcpy(dst, &r, 4);
         break;
     }
     case ORC_F64: {
-        double a = *(const double *)alpha, b = *(const double *)beta, s, d;
+        uint64_t a, b, s, d;
+        memcpy(&a, alpha, 8);
+        memcpy(&b, beta, 8);
         memcpy(&s, src, 8);
         memcpy(&d, dst, 8);
-        double bd = b * d;
-        double as = a * s;
-        double r = bd + as;
+        uint64_t bd = (ord & ORD_M1S) ? mul64(b, d) : mul64(d, b);
+        uint64_t as = (ord & ORD_M2S) ? mul64(a, s) : mul64(s, a);
+        uint64_t r = (ord & ORD_ARE) ? add64(as, bd) : add64(bd, as);
         memcpy(dst, &r, 8);
         break;
     }
     case ORC_C64: {
-        const float *a = (const float *)alpha, *b = (const float *)beta;
-        float s[2], d[2];
+        uint32_t a[2], b[2], s[2], d[2], bd[2], as[2], r[2];
+        memcpy(a, alpha, 8);
+        memcpy(b, beta, 8);
         memcpy(s, src, 8);
         memcpy(d, dst, 8);
-        /* complex product (ac - bd, ad + bc) */
-        float bd_re = b[0] * d[0] - b[1] * d[1];
-        float bd_im = b[0] * d[1] + b[1] * d[0];
-        float as_re = a[0] * s[0] - a[1] * s[1];
-        float as_im = a[0] * s[1] + a[1] * s[0];
-        float r[2] = {bd_re + as_re, bd_im + as_im};
+        cmul32(b, d, !(ord & ORD_M1S), bd);
+        cmul32(a, s, !(ord & ORD_M2S), as);
+        r[0] = (ord & ORD_ARE) ? add32(as[0], bd[0]) : add32(bd[0], as[0]);
+        r[1] = (ord & ORD_AIM) ? add32(as[1], bd[1]) : add32(bd[1], as[1]);
         memcpy(dst, r, 8);
         break;
     }
     case ORC_C128: {
-        const double *a = (const double *)alpha, *b = (const double *)beta;
-        double s[2], d[2];
+        uint64_t a[2], b[2], s[2], d[2], bd[2], as[2], r[2];
+        memcpy(a, alpha, 16);
+        memcpy(b, beta, 16);
         memcpy(s, src, 16);
         memcpy(d, dst, 16);
-        double bd_re = b[0] * d[0] - b[1] * d[1];
-        double bd_im = b[0] * d[1] + b[1] * d[0];
-        double as_re = a[0] * s[0] - a[1] * s[1];
-        double as_im = a[0] * s[1] + a[1] * s[0];
-        double r[2] = {bd_re + as_re, bd_im + as_im};
+        cmul64(b, d, !(ord & ORD_M1S), bd);
+        cmul64(a, s, !(ord & ORD_M2S), as);
+        r[0] = (ord & ORD_ARE) ? add64(as[0], bd[0]) : add64(bd[0], as[0]);
+        r[1] = (ord & ORD_AIM) ? add64(as[1], bd[1]) : add64(bd[1], as[1]);
         memcpy(dst, r, 16);
         break;
     }
     }
+}
+
+/* Operand order of the reference's scaled loops, per call site and
+ * position (what its compiled loops do; pinned against oracle/_ref):
+ *   COL   copy_and_scale over one column of n elements (copy2D_and_scale's
+ *         strided branch, memory_utils.hpp:113-120): real types put the
+ *         element first in both products; the sum takes beta*dst first in the
+ *         vector body (i < n - n % (16/sizeof(T))) and for n == 1, alpha*src
+ *         first in the remainder;
+ *   FLAT  copy_and_scale over a whole contiguous piece (:108-110): real types
+ *         take element pairs with the element first in both products and the
+ *         last element of an odd n with the scalar first; alpha*src first in
+ *         the sum;
+ *   TRANS copy_transpose_and_scale (:186-256): element first in both
+ *         products, alpha*src first in the sum; complex types: the diagonal
+ *         tiles (block_i == block_j) pass (alpha, el) to __mul?c3, the others
+ *         (el, alpha).
+ * Complex types otherwise pass (beta, dst) and (alpha, src) to __mul?c3 and
+ * put beta*dst first in both parts of the sum. */
+enum { SITE_COL = 0, SITE_FLAT = 1, SITE_TRANS = 2 };
+
+/* test hook: XORed into every operand order, so that a test can plant a
+ * wrong order and see the comparison with the reference fail */
+static int g_order_xor = 0;
+void orc_debug_order_xor(int x) { g_order_xor = x; }
+
+static int ref_order_site(int dtype, int site, int64_t i, int64_t n, int diag);
+static int ref_order(int dtype, int site, int64_t i, int64_t n, int diag) {
+    return ref_order_site(dtype, site, i, n, diag) ^ g_order_xor;
+}
+
+static int ref_order_site(int dtype, int site, int64_t i, int64_t n, int diag) {
+    const int real = dtype == ORC_F32 || dtype == ORC_F64;
+    if (site == SITE_COL) {
+        if (!real) return ORD_M1S | ORD_M2S;
+        const int64_t vf = dtype == ORC_F32 ? 4 : 2;
+        return (n == 1 || i < n - n % vf) ? 0 : ORD_ARE;
+    }
+    if (site == SITE_FLAT) {
+        if (!real) return ORD_M1S | ORD_M2S;
+        return ((n & 1) && i == n - 1) ? (ORD_M1S | ORD_M2S | ORD_ARE) : ORD_ARE;
+    }
+    if (real) return ORD_ARE;
+    if (diag) return ORD_M1S | ORD_M2S;
+    return dtype == ORC_C64 ? (ORD_M1S | ORD_ARE) : ORD_M1S;
 }
 
 /* alpha == T{1} && beta == T{0}: the exact-copy shortcut (memory_utils.hpp:32-35). */
@@ -389,30 +641,39 @@ static int is_plain_copy(int dtype, const void *alpha, const void *beta) {
     return 1;
 }
 
-/* memory::copy2D / copy2D_and_scale (memory_utils.hpp:41-122), column-major. */
+/* memory::copy2D / copy2D_and_scale (memory_utils.hpp:41-122), column-major:
+ * one copy of the whole piece when both leading dimensions equal the rows,
+ * else column by column. */
 static void copy2d(int dtype, size_t es, int64_t rows, int64_t cols, const unsigned char *src,
                    int64_t ld_src, unsigned char *dst, int64_t ld_dst, int plain,
                    const void *alpha, const void *beta) {
+    const int flat = rows == ld_src && rows == ld_dst;
     for (int64_t c = 0; c < cols; ++c) {
         const unsigned char *s = src + (size_t)(c * ld_src) * es;
         unsigned char *d = dst + (size_t)(c * ld_dst) * es;
         if (plain) {
             memcpy(d, s, (size_t)rows * es);
         } else {
-            for (int64_t r = 0; r < rows; ++r) scale_el(dtype, d + r * es, s + r * es, alpha, beta);
+            for (int64_t r = 0; r < rows; ++r) {
+                const int ord = flat ? ref_order(dtype, SITE_FLAT, c * rows + r, rows * cols, 0)
+                                     : ref_order(dtype, SITE_COL, r, rows, 0);
+                scale_el(dtype, d + r * es, s + r * es, alpha, beta, ord);
+            }
         }
     }
 }
 
 /* memory::copy_and_transpose / copy_transpose_and_scale (memory_utils.hpp:124-256):
  * dest[i*dest_stride + j] = op(src[j*src_stride + i]) for storage rows i <
- * n_rows and storage cols j < n_cols. The reference's tiling and diagonal
- * staging buffer only change the visiting order, not the result. */
+ * n_rows and storage cols j < n_cols. The reference's tiles of 512/sizeof(T)
+ * and its diagonal staging buffer change the visiting order, not the result,
+ * except for which operand the scaled epilogue takes first (ref_order). */
 static void copy_transpose(int dtype, size_t es, int64_t n_rows, int64_t n_cols,
                            const unsigned char *src, int64_t src_stride, unsigned char *dst,
                            int64_t dst_stride, int conj, int plain, const void *alpha,
                            const void *beta) {
     unsigned char el[16];
+    const int64_t tile = 512 / (int64_t)es;
     for (int64_t i = 0; i < n_rows; ++i) {
         for (int64_t j = 0; j < n_cols; ++j) {
             memcpy(el, src + (size_t)(j * src_stride + i) * es, es);
@@ -421,7 +682,7 @@ static void copy_transpose(int dtype, size_t es, int64_t n_rows, int64_t n_cols,
             if (plain)
                 memcpy(d, el, es);
             else
-                scale_el(dtype, d, el, alpha, beta);
+                scale_el(dtype, d, el, alpha, beta, ref_order(dtype, SITE_TRANS, 0, 0, i / tile == j / tile));
         }
     }
 }
@@ -735,6 +996,63 @@ void orc_value(int dtype, uint64_t seed, int64_t i, int64_t j, int64_t n_cols, v
         break;
     }
     }
+}
+
+/* grid_layout::scale_by (grid_layout.hpp:48-56) and block::scale_by
+ * (block.cpp:291-306): nothing when beta == 1, else every element of every
+ * block times beta, data[offset] *= beta. The reference's build takes the
+ * element as the first source of a real product and passes (beta, element)
+ * to __mul?c3 for a complex one, at every position. */
+int orc_scale_layout(const orc_layout *l, int dtype, const void *beta) {
+    size_t es = orc_dtype_size(dtype);
+    if (!es) return -1;
+    static const unsigned char zero[16] = {0};
+    if (is_plain_copy(dtype, beta, zero)) return 0;  /* beta == T{1} */
+    const int swap = (g_order_xor & ORD_M1S) != 0;  /* planted mistake (tests) */
+    for (int b = 0; b < l->n_blocks; ++b) {
+        const orc_block *blk = &l->blocks[b];
+        int R0 = l->rows_split[blk->row], R1 = l->rows_split[blk->row + 1];
+        int C0 = l->cols_split[blk->col], C1 = l->cols_split[blk->col + 1];
+        for (int j = C0; j < C1; ++j)
+            for (int i = R0; i < R1; ++i) {
+                unsigned char *x = (unsigned char *)blk->data + (size_t)((j - C0) * blk->stride + (i - R0)) * es;
+                switch (dtype) {
+                case ORC_F32: {
+                    uint32_t v, bb;
+                    memcpy(&v, x, 4);
+                    memcpy(&bb, beta, 4);
+                    v = swap ? mul32(bb, v) : mul32(v, bb);
+                    memcpy(x, &v, 4);
+                    break;
+                }
+                case ORC_F64: {
+                    uint64_t v, bb;
+                    memcpy(&v, x, 8);
+                    memcpy(&bb, beta, 8);
+                    v = swap ? mul64(bb, v) : mul64(v, bb);
+                    memcpy(x, &v, 8);
+                    break;
+                }
+                case ORC_C64: {
+                    uint32_t v[2], bb[2], o[2];
+                    memcpy(v, x, 8);
+                    memcpy(bb, beta, 8);
+                    cmul32(bb, v, swap, o);
+                    memcpy(x, o, 8);
+                    break;
+                }
+                default: {
+                    uint64_t v[2], bb[2], o[2];
+                    memcpy(v, x, 16);
+                    memcpy(bb, beta, 16);
+                    cmul64(bb, v, swap, o);
+                    memcpy(x, o, 16);
+                    break;
+                }
+                }
+            }
+    }
+    return 0;
 }
 
 int orc_fill_layout(const orc_layout *l, int dtype, uint64_t seed) {

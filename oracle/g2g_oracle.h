@@ -172,6 +172,23 @@ int orc_fill_layout(const orc_layout *l, int dtype, uint64_t seed);
 int64_t orc_check_layout(const orc_layout *fin, int dtype, uint64_t seed, char trans,
                          int64_t a_n_cols);
 
+/* grid_layout::scale_by / block::scale_by (grid_layout.hpp:48-56,
+ * block.cpp:291-306): every element of every block of l times beta, unless
+ * beta == 1. */
+int orc_scale_layout(const orc_layout *l, int dtype, const void *beta);
+
+/* Scaled epilogue internals, for tests. orc_cmul: the restated std::complex
+ * product l*r of complex<float> (ORC_C64) or complex<double> (ORC_C128), with
+ * __mul?c3 taking r's parts first when swap is set; orc_libgcc_mulxc3: libgcc's
+ * own __mulsc3 / __muldc3 on l's and r's parts (what the restatement of C11
+ * Annex G.5.1 is checked against). orc_debug_order_xor: XORed into every
+ * operand order of the scaled epilogue, its ORD_M1S bit also swaps the
+ * operands of scale_by (0 = as the reference; a test plants a wrong order
+ * with it). */
+void orc_cmul(int dtype, const void *l, const void *r, int swap, void *out);
+void orc_libgcc_mulxc3(int dtype, const void *l, const void *r, void *out);
+void orc_debug_order_xor(int x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -277,6 +277,23 @@ int ref_comm_volume(const orc_layout *init, const orc_layout *fin, int64_t *vol)
     }
 }
 
+// grid_layout::scale_by (grid_layout.hpp:48-56, restated: that header
+// includes <mpi.h>): unless beta == 1, the reference's own block::scale_by
+// (block.cpp:291-306) on every block of the layout.
+int ref_scale_layout(const orc_layout *l, int dtype, const void *beta) {
+    return dispatch(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const T b = load_scalar<T>(beta, T{1});
+        if (b == T{1}) return 0;
+        assigned_grid2D g = make_grid(l);
+        for (int r = 0; r < l->n_ranks; ++r) {
+            local_blocks<T> blocks = make_blocks<T>(l, g, r);
+            for (int k = 0; k < blocks.num_blocks(); ++k) blocks.get_block(k).scale_by(b);
+        }
+        return 0;
+    });
+}
+
 // ranks_reordering.cpp:4-61, the reference's own optimal_reordering, on a
 // comm_volume holding the nonzero entries of the dense upper triangle.
 int ref_optimal_reordering(const int64_t *vol, int n, int32_t *perm, int *reordered) {

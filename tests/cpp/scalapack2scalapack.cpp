@@ -116,11 +116,40 @@ static bool run_case(int m, int n, int ibm, int ibn, int fbm, int fbn, int pm, i
         verify();
         reset();
     }
+    int64_t hits0 = 0;
+    detail::check(g2g_plan_cache_stats(&hits0, nullptr, nullptr));
     if (trans == 'N')
         transform<double>(init, fin);
     else
         transform<double>(init, fin, trans, 1.0, 0.0);
     verify();
+    // the same layout pair again: served from the plan cache
+    if (trans == 'N') {
+        reset();
+        transform<double>(init, fin);
+        verify();
+        int64_t hits1 = 0;
+        detail::check(g2g_plan_cache_stats(&hits1, nullptr, nullptr));
+        if (hits1 != hits0 + 1) {
+            std::printf("plan cache: %lld hits after a repeated pair, %lld before\n", (long long)hits1,
+                        (long long)hits0);
+            ok = false;
+        }
+    }
+    // the plan's launch replayed from its hipGraph, twice
+    if (tuned) {
+        hipStream_t st = nullptr;
+        HIPCHECK(hipStreamCreate(&st));
+        transform_plan<double> plan(init, fin, trans, st);
+        virtual_comm comm;
+        comm.stream = st;
+        for (int rep = 0; rep < 2; ++rep) {
+            reset();
+            plan.execute_graph(1.0, 0.0, comm);
+            verify();
+        }
+        HIPCHECK(hipStreamDestroy(st));
+    }
     std::printf("m=%d n=%d init %dx%d final %dx%d grid %dx%d trans %c: %s\n", m, n, ibm, ibn, fbm, fbn, pm, pn,
                 trans, ok ? "correct" : "NOT correct");
     return ok;
@@ -264,6 +293,55 @@ static bool transformer_case() {
     return ok;
 }
 
+// grid_layout::scale_by and block::scale_by / local_element on device
+// memory (grid_layout.hpp:48-56, block.cpp:228-306): every element doubled
+// (exact), padding untouched, one element read and written through the proxy.
+static bool scale_case() {
+    const int P = 4;
+    scalapack::data_layout l({30, 20}, {4, 3}, {2, 2}, scalapack::column_major);
+    rank_buffers bufs;
+    std::vector<grid_layout<double>> per;
+    std::vector<std::vector<double>> host;
+    for (int r = 0; r < P; ++r) {
+        host.push_back(local_values(l, r, cantor));
+        double *d = nullptr;
+        HIPCHECK(hipMalloc(&d, sizeof(double) * (host[r].size() + 1)));
+        HIPCHECK(hipMemcpy(d, host[r].data(), sizeof(double) * host[r].size(), hipMemcpyHostToDevice));
+        bufs.dev.push_back(d);
+        bufs.size.push_back(host[r].size());
+        per.push_back(get_scalapack_grid<double>(l, d, r));
+    }
+    grid_layout<double> lay = merge_ranks(per);
+    lay.scale_by(2.0);
+    lay.scale_by(1.0);  // no-op
+    bool ok = true;
+    for (int r = 0; r < P; ++r) {
+        std::vector<double> got(bufs.size[r]);
+        HIPCHECK(hipMemcpy(got.data(), bufs.dev[r], sizeof(double) * got.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < got.size(); ++k)
+            ok &= got[k] == (host[r][k] == -7.0 ? -7.0 : 2.0 * host[r][k]);
+    }
+    block<double> &b = lay.blocks.get_block(3);
+    const int gi = b.rows_interval.start + 1, gj = b.cols_interval.start + 2;
+    ok &= double(b.local_element(1, 2)) == 2.0 * cantor(gi, gj);
+    b.local_element(1, 2) = 0.5;
+    ok &= double(b.local_element(1, 2)) == 0.5;
+    b.scale_by(4.0);
+    ok &= double(b.local_element(1, 2)) == 2.0 && double(b.local_element(0, 0)) ==
+          8.0 * cantor(b.rows_interval.start, b.cols_interval.start);
+    bool refused = false;
+    block<double> t = b;
+    t.transpose_or_conjugate('T');
+    try {
+        t.scale_by(3.0);
+    } catch (const std::runtime_error &) {
+        refused = true;
+    }
+    ok &= refused;
+    std::printf("scale_by / local_element: %s\n", ok ? "correct" : "NOT correct");
+    return ok;
+}
+
 // communication_volume against a count over elements, and optimal_reordering:
 // relabelling the final grid's ranks with the returned permutation lowers the
 // remote volume by exactly the adjusted weights of the matched pairs
@@ -318,6 +396,7 @@ int main(int argc, char **argv) {
     ok &= run_case(700, 500, 24, 40, 56, 32, 2, 2, 'N', true);
     ok &= transformer_case();
     ok &= volume_case();
+    ok &= scale_case();
     std::printf("%s\n", ok ? "ALL CORRECT" : "FAILURES");
     return !ok;
 }

@@ -53,6 +53,9 @@ def oracle():
         lib.orc_check_layout.argtypes = [vp, i, ctypes.c_uint64, ctypes.c_char, i64]
         lib.orc_comm_volume.argtypes = [vp, vp, p64]
         lib.orc_optimal_reordering.argtypes = [p64, i, p32, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_cmul.argtypes = [i, vp, vp, i, vp]
+        lib.orc_libgcc_mulxc3.argtypes = [i, vp, vp, vp]
+        lib.orc_debug_order_xor.argtypes = [i]
         _oracle = lib
     return _oracle
 
@@ -249,3 +252,83 @@ def optimal_reordering(lib_fn, vol):
     rc = lib_fn(i64p(vol), n, i32p(perm), ctypes.byref(r))
     assert rc == 0, "optimal_reordering failed"
     return [int(x) for x in perm], bool(r.value)
+
+
+# Bit patterns the scaled epilogue must treat exactly as the reference's x86
+# build does: quiet and signalling NaNs with payloads and both signs, the
+# infinities, signed zeros, the smallest and largest denormals, 1, the largest
+# finite value, the smallest normal and values whose products overflow.
+SPECIAL32 = np.array([0x7fc00000, 0xffc00000, 0x7f800001, 0x7fa12345, 0xffb00001, 0x7fc0beef,
+                      0x7f800000, 0xff800000, 0x00000000, 0x80000000, 0x00000001, 0x807fffff,
+                      0x3f800000, 0xbf800000, 0x7f7fffff, 0x00800000, 0x5f000000, 0xdf000000],
+                     dtype=np.uint32)
+SPECIAL64 = np.array([0x7ff8000000000000, 0xfff8000000000000, 0x7ff0000000000001, 0x7ff4123456789abc,
+                      0xfff0000000000005, 0x7ff8dead0000beef, 0x7ff0000000000000, 0xfff0000000000000,
+                      0, 0x8000000000000000, 1, 0x800fffffffffffff, 0x3ff0000000000000,
+                      0xbff0000000000000, 0x7fefffffffffffff, 0x0010000000000000, 0x5ff0000000000000,
+                      0xdff0000000000000], dtype=np.uint64)
+
+
+def special_arena(dtype, n_elems, rng, frac=0.6):
+    """n_elems elements of dtype as bytes: random bit patterns, `frac` of the
+    components replaced by SPECIAL32/SPECIAL64 entries."""
+    es = ES[dtype]
+    if dtype in (F32, C64):
+        k = n_elems * es // 4
+        v = rng.integers(0, 2**32, size=k, dtype=np.uint64).astype(np.uint32)
+        pool = SPECIAL32
+    else:
+        k = n_elems * es // 8
+        v = rng.integers(0, 2**63, size=k, dtype=np.uint64) * 2 + rng.integers(0, 2, size=k, dtype=np.uint64)
+        pool = SPECIAL64
+    pick = rng.random(k) < frac
+    v[pick] = pool[rng.integers(0, len(pool), size=int(pick.sum()))]
+    return v.view(np.uint8)
+
+
+def special_scalar(dtype, rng, p_special=0.7):
+    """One alpha or beta of dtype: special components with probability p_special."""
+    pool = SPECIAL32 if dtype in (F32, C64) else SPECIAL64
+    n = 1 if dtype in (F32, F64) else 2
+    if rng.random() < p_special:
+        v = pool[rng.integers(0, len(pool), size=n)]
+    else:
+        v = rng.integers(0, 2**31, size=n).astype(pool.dtype) << (0 if pool.dtype == np.uint32 else 32)
+    return v.view(NP_DTYPE[dtype])[0]
+
+
+def cmul(dtype, l, r, swap=0):
+    """The oracle's complex product of two complex scalars as bytes."""
+    es = ES[dtype]
+    a = np.array([l], dtype=NP_DTYPE[dtype])
+    b = np.array([r], dtype=NP_DTYPE[dtype])
+    out = np.zeros(es, dtype=np.uint8)
+    oracle().orc_cmul(dtype, a.ctypes.data, b.ctypes.data, swap, out.ctypes.data)
+    return out
+
+
+def libgcc_mulxc3(dtype, l, r):
+    es = ES[dtype]
+    a = np.array([l], dtype=NP_DTYPE[dtype])
+    b = np.array([r], dtype=NP_DTYPE[dtype])
+    out = np.zeros(es, dtype=np.uint8)
+    oracle().orc_libgcc_mulxc3(dtype, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def scale_layout(layout, arena, dtype, beta):
+    """Oracle grid_layout::scale_by on a host arena."""
+    d = layout.descriptor(arena.ctypes.data, ES[dtype])
+    b = np.array([beta], dtype=NP_DTYPE[dtype])
+    lib = oracle()
+    lib.orc_scale_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert lib.orc_scale_layout(ctypes.addressof(d.struct), dtype, b.ctypes.data) == 0
+
+
+def ref_scale_layout(layout, arena, dtype, beta):
+    """The reference's block::scale_by over every block of a host arena."""
+    d = layout.descriptor(arena.ctypes.data, ES[dtype])
+    b = np.array([beta], dtype=NP_DTYPE[dtype])
+    lib = ref()
+    lib.ref_scale_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert lib.ref_scale_layout(ctypes.addressof(d.struct), dtype, b.ctypes.data) == 0

@@ -9,7 +9,8 @@ from ._lib import (FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, DTYPE_SIZE, DTYPE_NA
 from .layout import Layout, scalapack_layout, arbitrary_layout, line_split, random_splits
 from .api import (Plan, Batch, Server, SERVE_BATCHED, SERVE_PLAIN, set_tuning, fill_layout,
                   verify_layout, comm_volume, optimal_reordering, stream_probe, transform_layouts,
-                  plan_cache_stats, plan_cache_clear, plan_cache_set_capacity, scale_layout)
+                  plan_cache_stats, plan_cache_clear, plan_cache_set_capacity, scale_layout,
+                  transform_plans_noalloc)
 
 __all__ = [
     "FLOAT32", "FLOAT64", "COMPLEX64", "COMPLEX128", "DTYPE_SIZE", "DTYPE_NAMES", "G2GError",
@@ -17,4 +18,5 @@ __all__ = [
     "random_splits", "Plan", "Batch", "Server", "SERVE_BATCHED", "SERVE_PLAIN", "set_tuning",
     "fill_layout", "verify_layout", "comm_volume", "optimal_reordering", "stream_probe",
     "transform_layouts", "plan_cache_stats", "plan_cache_clear", "plan_cache_set_capacity", "scale_layout",
+    "transform_plans_noalloc",
 ]

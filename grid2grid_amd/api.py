@@ -194,6 +194,23 @@ class Batch:
             pass
 
 
+def transform_plans_noalloc(plans, alphas=None, betas=None, stream=None):
+    """Several plans in one launch with no device allocation
+    (g2g_transform_plans_noalloc); calls that overlap on one GPU race."""
+    n = len(plans)
+    arr = (ctypes.c_void_p * max(1, n))(*[p.handle for p in plans])
+    keep = []
+    a_arr = b_arr = None
+    if alphas is not None:
+        a_bufs = [_scalar_bytes(p.dtype, a) for p, a in zip(plans, alphas)]
+        b_bufs = [_scalar_bytes(p.dtype, b) for p, b in zip(plans, betas)]
+        keep = a_bufs + b_bufs
+        a_arr = (ctypes.c_void_p * n)(*[ctypes.cast(x, ctypes.c_void_p) if x is not None else None for x in a_bufs])
+        b_arr = (ctypes.c_void_p * n)(*[ctypes.cast(x, ctypes.c_void_p) if x is not None else None for x in b_bufs])
+    check(_lib.load().g2g_transform_plans_noalloc(n, arr, a_arr, b_arr, _stream(stream)))
+    del keep
+
+
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
 

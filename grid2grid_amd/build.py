@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libgrid2grid_amd.so")
 # (workgroup size, feature level) family so that they build in parallel
 SOURCES = [("g2g_kernels.hip", "g2g_kernels", []), ("g2g_host.cpp", "g2g_host", []),
            ("g2g_serve.cpp", "g2g_serve", []), ("g2g_rank_pairs.hip", "g2g_rank_pairs", []),
-           ("g2g_cache.cpp", "g2g_cache", [])] + [
+           ("g2g_cache.cpp", "g2g_cache", []), ("g2g_noalloc.hip", "g2g_noalloc", [])] + [
     ("g2g_gather_variants.hip", f"g2g_gather_{th}_{feat}", [f"-DG2G_THREADS={th}", f"-DG2G_FEAT={feat}"])
     for th in (256, 512) for feat in (0, 1, 2)]
 

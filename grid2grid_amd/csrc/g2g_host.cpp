@@ -1251,6 +1251,21 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
 
 namespace g2g {
 
+int internal_desc(g2g_plan_s *p, const void *alpha, const void *beta, uint32_t tile_base, work_desc &out,
+                  launch_cfg &cfg) {
+    if (!p) return fail(G2G_ERR_INVALID, "plan is NULL");
+    if ((alpha == nullptr) != (beta == nullptr)) return fail(G2G_ERR_INVALID, "alpha/beta mismatch");
+    kreqs kr;
+    std::memset(&kr, 0, sizeof kr);
+    kr.r[0] = make_request(p->dtype, p->trans, alpha, beta, true, true);
+    kr.r[1] = kr.r[0];
+    const stage_table &t = xtab(p, kr);
+    out = make_desc(t, kr, tile_base);
+    cfg = t.cfg;
+    cfg.features = request_features(kr.r[0]);
+    return G2G_OK;
+}
+
 int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas, const void *const *betas,
                    std::vector<work_desc> &out, uint32_t &total_tiles, uint64_t &total_bytes) {
     out.clear();

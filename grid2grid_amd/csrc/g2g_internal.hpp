@@ -231,6 +231,11 @@ hipError_t launch_work_records(const work_desc *descs, int32_t n, work_rec *work
 int internal_descs(int n, g2g_plan_s *const *plans, const void *const *alphas,
                    const void *const *betas, std::vector<work_desc> &out, uint32_t &total_tiles,
                    uint64_t &total_bytes);
+// The work descriptor of one plan's request (work items numbered from
+// tile_base) and the plan's launch configuration with the request's feature
+// level. Allocates nothing. Returns a g2g_status.
+int internal_desc(g2g_plan_s *p, const void *alpha, const void *beta, uint32_t tile_base, work_desc &out,
+                  launch_cfg &cfg);
 // Compute units of the device (persistent grids are sized from it).
 int internal_grid_limit();
 // Records the thread's last error message and returns code.

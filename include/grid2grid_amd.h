@@ -241,6 +241,23 @@ int g2g_batch_bytes(g2g_batch batch, int64_t *bytes);
 
 int g2g_batch_destroy(g2g_batch batch);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops that must not call
+ * hipMalloc). The work of n plans (alphas/betas as for g2g_batch_create) in
+ * one persistent gather launch on `stream`, like g2g_batch_create followed
+ * by g2g_batch_execute, but the launch's descriptors, per-work-item records
+ * and work counters live in one 4 MiB device array of the library (one per
+ * GPU, held until the process ends) instead of allocations: the call
+ * allocates nothing and copies nothing from the host (descriptors travel as
+ * kernel arguments), so it can be captured. Each plan runs over its own work
+ * items (its g2g_plan_get_tuning tile size). At most 4096 plans and 74401
+ * work items per call; larger requests fail with G2G_ERR_INVALID before
+ * anything is launched. Every call on a GPU shares that GPU's array and
+ * leaves its contents behind: calls that overlap on one GPU (other streams,
+ * other threads, replays of graphs holding such a call) race; order them. */
+int g2g_transform_plans_noalloc(int n, const g2g_plan *plans, const void *const *alphas,
+                                const void *const *betas, void *stream);
+
 /* ---------------------------------------------------------------- serving */
 
 /* A request loop on its own host thread: transform requests (plans whose

@@ -283,10 +283,20 @@ def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev)
         busy = {}  # pool index -> ticket in flight
         pools = {True: list(range(len(large))), False: list(range(len(small)))}
         nxt = {True: 0, False: 0}
-        lat, done_bytes = [], 0
+        lat_of = {}  # ticket -> latency, once the server reported it (each ticket is reported once)
+        done_bytes = 0
         t_start = time.perf_counter()
         t_next = t_start
         tickets = []
+
+        def finished(t):
+            if t not in lat_of:
+                x = server.poll(t)
+                if x is None:
+                    return False
+                lat_of[t] = x
+            return True
+
         for i in range(n_req):
             big = bool(kinds[i])
             pool = large if big else small
@@ -296,10 +306,11 @@ def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev)
                 nxt[big] += 1
                 key = (big, j)
                 t = busy.get(key)
-                if t is None or server.poll(t) is not None:
+                if t is None or finished(t):
                     break
             else:
-                server.wait(busy[key])
+                if busy[key] not in lat_of:
+                    lat_of[busy[key]] = server.wait(busy[key])
             if open_loop:
                 t_next += gaps[i]
                 while time.perf_counter() < t_next:
@@ -308,9 +319,11 @@ def served_leg(torch, g2g, workloads, n_small, n_large, n_req, qps, budget, dev)
             busy[key] = t
             tickets.append((t, pool[j]["bytes"]))
         for t, b in tickets:
-            lat.append(server.wait(t))
+            if t not in lat_of:
+                lat_of[t] = server.wait(t)
             done_bytes += b
         wall = time.perf_counter() - t_start
+        lat = [lat_of[t] for t, _ in tickets]
         return lat, done_bytes, wall
 
     res = {}

@@ -19,6 +19,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 using namespace g2g;
@@ -51,6 +52,7 @@ struct slot {
 struct result {
     double latency = 0;
     int status = G2G_OK;
+    std::string error;  // this request's own failure
 };
 
 }  // namespace
@@ -64,7 +66,8 @@ struct g2g_server_s {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<pending_req> pending;
-    std::unordered_map<int64_t, result> finished;
+    std::unordered_map<int64_t, result> finished;  // until poll or wait reports them
+    std::unordered_set<int64_t> unreported;         // submitted, not yet reported by poll or wait
     int64_t next_ticket = 1;
     int64_t in_flight = 0;
     int busy_slots = 0;
@@ -171,6 +174,7 @@ void g2g_server_s::loop() {
                 result res;
                 res.latency = std::chrono::duration<double>(now - r.submitted).count();
                 res.status = q == hipSuccess ? G2G_OK : G2G_ERR_HIP;
+                if (q != hipSuccess) res.error = std::string("server: launch failed: ") + hipGetErrorString(q);
                 finished[r.ticket] = res;
                 in_flight -= r.footprint;
                 st.completed++;
@@ -215,6 +219,7 @@ void g2g_server_s::loop() {
                         result res;
                         res.latency = std::chrono::duration<double>(now - r.submitted).count();
                         res.status = rc;
+                        res.error = msg;
                         finished[r.ticket] = res;
                         in_flight -= r.footprint;
                     }
@@ -292,19 +297,28 @@ int g2g_server_submit(g2g_server s, g2g_plan plan, const void *alpha, const void
     r.ticket = s->next_ticket++;
     r.submitted = clock_type::now();
     *ticket = r.ticket;
+    s->unreported.insert(r.ticket);
     s->pending.push_back(r);
     s->cv_work.notify_one();
     return G2G_OK;
 }
 
+// Reports a finished ticket once: its entry is erased, so a long-running
+// server keeps only results nobody has collected yet.
 int g2g_server_poll(g2g_server s, int64_t ticket, int32_t *done, double *latency_s) {
     if (!s || !done) return internal_fail(G2G_ERR_INVALID, "server: bad arguments");
     std::lock_guard<std::mutex> g(s->mu);
+    if (ticket <= 0 || ticket >= s->next_ticket) return internal_fail(G2G_ERR_INVALID, "server: unknown ticket");
+    if (!s->unreported.count(ticket))
+        return internal_fail(G2G_ERR_STATE, "server: ticket %lld was already reported", (long long)ticket);
     auto it = s->finished.find(ticket);
     *done = it != s->finished.end();
     if (*done) {
-        if (latency_s) *latency_s = it->second.latency;
-        if (it->second.status != G2G_OK) return internal_fail(it->second.status, "%s", s->error.c_str());
+        const result r = it->second;
+        s->finished.erase(it);
+        s->unreported.erase(ticket);
+        if (latency_s) *latency_s = r.latency;
+        if (r.status != G2G_OK) return internal_fail(r.status, "%s", r.error.c_str());
     }
     return G2G_OK;
 }
@@ -313,10 +327,15 @@ int g2g_server_wait(g2g_server s, int64_t ticket, double *latency_s) {
     if (!s) return internal_fail(G2G_ERR_INVALID, "server: bad arguments");
     std::unique_lock<std::mutex> lk(s->mu);
     if (ticket <= 0 || ticket >= s->next_ticket) return internal_fail(G2G_ERR_INVALID, "server: unknown ticket");
+    if (!s->unreported.count(ticket))
+        return internal_fail(G2G_ERR_STATE, "server: ticket %lld was already reported", (long long)ticket);
     s->cv_done.wait(lk, [&] { return s->finished.count(ticket) != 0; });
-    const result &r = s->finished[ticket];
+    auto it = s->finished.find(ticket);
+    const result r = it->second;
+    s->finished.erase(it);
+    s->unreported.erase(ticket);
     if (latency_s) *latency_s = r.latency;
-    if (r.status != G2G_OK) return internal_fail(r.status, "%s", s->error.c_str());
+    if (r.status != G2G_OK) return internal_fail(r.status, "%s", r.error.c_str());
     return G2G_OK;
 }
 

@@ -294,10 +294,13 @@ int g2g_server_create(int64_t budget_bytes, int n_streams, int max_batch, int mo
 int g2g_server_submit(g2g_server s, g2g_plan plan, const void *alpha, const void *beta,
                       int64_t *ticket);
 
-/* *done = 1 and *latency_s = completion - submission time once finished. */
+/* *done = 1 and *latency_s = completion - submission time once finished,
+ * with the request's own status (a failed request returns its error). A
+ * finished ticket is reported once, by poll or wait: the server then forgets
+ * it, and asking again fails with G2G_ERR_STATE. */
 int g2g_server_poll(g2g_server s, int64_t ticket, int32_t *done, double *latency_s);
 
-/* Block until the ticket finished. */
+/* Block until the ticket finished; reports it as g2g_server_poll does. */
 int g2g_server_wait(g2g_server s, int64_t ticket, double *latency_s);
 
 /* Block until every submitted request finished. */

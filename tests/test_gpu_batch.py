@@ -139,6 +139,30 @@ def test_server_batched_results_equal_alone():
         c.check(f"served request {k}")
 
 
+def test_server_reports_each_ticket_once():
+    """poll and wait report a finished ticket once, then forget it: asking
+    again is an error, and an unknown ticket is refused."""
+    cases = _cases(3, 6)
+    srv = g2g.Server(1 << 40, n_streams=2, max_batch=4)
+    t0 = srv.submit(cases[0].plan)
+    t1 = srv.submit(cases[1].plan)
+    assert srv.wait(t0) >= 0
+    with pytest.raises(g2g.G2GError) as e:
+        srv.wait(t0)
+    assert e.value.code == 6
+    srv.drain()
+    assert srv.poll(t1) is not None
+    with pytest.raises(g2g.G2GError) as e:
+        srv.poll(t1)
+    assert e.value.code == 6
+    with pytest.raises(g2g.G2GError) as e:
+        srv.poll(12345)
+    assert e.value.code == 1
+    srv.close()
+    for k, c in enumerate(cases[:2]):
+        c.check(f"request {k}")
+
+
 def test_server_plain_mode_and_budget_reject():
     cases = _cases(6, 4)
     srv = g2g.Server(1 << 40, mode=g2g.SERVE_PLAIN)
@@ -185,11 +209,11 @@ def test_batch_refuses_after_a_plan_is_retuned():
 
 
 def test_batch_of_the_benchmark_distribution():
-    """512 requests drawn from the benchmark's mixed distribution at full
+    """2048 requests drawn from the benchmark's mixed distribution at full
     size (m, n up to 4096, f32/f64/c64, blocks 16..256, grids up to 4x4, 30%
     transposed) in one hipGraph-replayed launch: after each of two replays
     every request's final layout equals the seeded closed form."""
-    reqs = workloads.mixed_requests(n=512, seed=77)
+    reqs = workloads.mixed_requests(n=2048, seed=77)
     rs = []
     for k, (init, fin, dtype, trans) in enumerate(reqs):
         es = g2g.DTYPE_SIZE[dtype]
@@ -211,3 +235,65 @@ def test_batch_of_the_benchmark_distribution():
     batch.close()
     for r in rs:
         r[6].close()
+
+
+def test_served_stream_of_the_benchmark_mix():
+    """The served workload: a Poisson stream (2000 requests/s) of 300
+    requests of the mixed distribution and 16 large 16384^2 f64 requests
+    (128x128 on 4x4 -> 256x256 on 2x8; 5% of the stream) through the
+    continuous-batching server under the 64 GB HBM budget. Every result
+    equals the seeded closed form, the admitted footprint never exceeds the
+    budget, and a sample of requests run again alone gives the same bytes as
+    in the mix."""
+    import time
+    rng = np.random.default_rng(2005)
+    small = [workloads.random_request(rng) for _ in range(300)]
+    reqs = small[:]
+    for k in range(16):  # the large requests at spread positions
+        reqs.insert(int(k * len(reqs) / 16) + k, workloads.large_request())
+    assert sum(1 for r in reqs if r[0].m == 16384) == 16
+    rs = []
+    for k, (init, fin, dtype, trans) in enumerate(reqs):
+        es = g2g.DTYPE_SIZE[dtype]
+        ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device="cuda")
+        tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device="cuda")
+        g2g.fill_layout(init, ti.data_ptr(), dtype, 500 + k)
+        rs.append((init, fin, dtype, trans, ti, tf, g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)))
+    torch.cuda.synchronize()
+    budget = 64 * 10**9
+    srv = g2g.Server(budget, n_streams=3, max_batch=512)
+    gaps = rng.exponential(1.0 / 2000.0, size=len(rs))
+    tickets = []
+    t_next = time.perf_counter()
+    for k, r in enumerate(rs):
+        t_next += gaps[k]
+        while time.perf_counter() < t_next:
+            pass
+        tickets.append(srv.submit(r[6]))
+    lat = [srv.wait(t) for t in tickets]
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    print(f"served {st['completed']} requests in {st['launches']} launches, peak in flight "
+          f"{st['peak_in_flight_bytes'] / 1e9:.1f} GB, p50 {np.percentile(lat, 50) * 1e3:.2f} ms, "
+          f"p99 {np.percentile(lat, 99) * 1e3:.2f} ms", flush=True)
+    assert st["completed"] == len(rs) and st["rejected"] == 0
+    assert st["peak_in_flight_bytes"] <= budget
+    assert st["launches"] < len(rs)  # requests were batched together
+    bad = [k for k, (init, fin, dtype, trans, ti, tf, plan) in enumerate(rs)
+           if g2g.verify_layout(fin, tf.data_ptr(), dtype, 500 + k, trans, init.n) != 0]
+    assert not bad, f"requests {bad[:10]} differ"
+    # a sample again, alone: the same bytes as in the mix
+    sample = [k for k in range(len(rs)) if rs[k][0].m == 16384][:3] + list(range(0, len(rs), 37))
+    for k in sample:
+        init, fin, dtype, trans, ti, tf, plan = rs[k]
+        in_mix = tf.clone()
+        tf.zero_()
+        plan.transform()
+        torch.cuda.synchronize()
+        assert torch.equal(tf, in_mix), f"request {k}: alone differs from the mix"
+        del in_mix
+    for r in rs:
+        r[6].close()
+    del rs
+    torch.cuda.empty_cache()

@@ -96,6 +96,9 @@ def build_library(verbose=False, incremental=False):
     return LIB
 
 
+HOST_REPLAYS = {"gather_replay.cpp"}
+
+
 def build_cpp_tests(verbose=False):
     """C++ programs under tests/cpp written against the header-only facade
     (include/grid2grid), linked to the library; built into tests/cpp/bin."""
@@ -106,10 +109,16 @@ def build_cpp_tests(verbose=False):
     for f in sorted(os.listdir(src_dir)):
         if not f.endswith(".cpp"):
             continue
-        out = os.path.join(out_dir, f[:-4])
-        cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
-               os.path.join(src_dir, f), "-L" + HERE, "-lgrid2grid_amd",
-               "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../grid2grid_amd", "-o", out]
+        if f in HOST_REPLAYS:  # host-only replays of the kernels' index arithmetic: shared objects
+            out = os.path.join(out_dir, "lib" + f[:-4] + ".so")
+            cmd = [hipcc(), "-std=c++17", "-O2", "--offload-arch=gfx950", "-fPIC", "-shared",
+                   "-I" + os.path.join(ROOT, "include"),
+                   "-I" + CSRC, os.path.join(src_dir, f), "-o", out]
+        else:
+            out = os.path.join(out_dir, f[:-4])
+            cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+                   os.path.join(src_dir, f), "-L" + HERE, "-lgrid2grid_amd",
+                   "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../grid2grid_amd", "-o", out]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

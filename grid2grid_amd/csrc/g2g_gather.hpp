@@ -10,7 +10,7 @@
 // arithmetic on bit patterns.
 #pragma once
 
-#include "g2g_internal.hpp"
+#include "g2g_index.hpp"
 
 // no FMA contraction anywhere (the build also passes -ffp-contract=off)
 #pragma clang fp contract(off)
@@ -288,10 +288,6 @@ __device__ __forceinline__ u32x4 conj_bits(u32x4 v) {
 
 // ---------------------------------------------------------- copy paths
 
-__device__ __forceinline__ uint32_t ceil_log2(uint32_t v) {
-    return v <= 1 ? 0u : 32u - (uint32_t)__clz(v - 1);
-}
-
 // Global loads/stores, optionally with the non-temporal hint (streaming data
 // that is touched exactly once).
 template <bool NT, typename T>
@@ -375,50 +371,6 @@ __device__ __forceinline__ void vec_set(u32x4 &v, int e, typename bits_t<ES>::ty
     }
 }
 
-// Runs queued for one flat copy.
-// SEG_VEC: columns of colb bytes starting at s (source, leading dimension
-//   lds_b bytes) and d (destination, ldd_b), each cut into vpc = ceil(colb/16)
-//   16-byte slots; when colb is not a multiple of 16 the last slot is the
-//   column's last 16 bytes (overlapping the slot before it), or, in a column
-//   shorter than 16 bytes, its 1-3 dwords.
-// SEG_TRANS / SEG_CONJ (a queue of their own): a small transposed piece of
-//   ncols final columns: destination element (i, c) at d + c*ldd_b + i*es
-//   comes from s + i*lds_b + c*es, its imaginary sign bit flipped for
-//   SEG_CONJ. One slot per group of 16/es final columns and, for 4- and 8-byte
-//   elements, group of 16/es final rows (vpc = ceil(rows*es/16)), or final
-//   row (16-byte elements, vpc = final rows).
-// slot0 numbers the segment's first slot in its queue.
-enum : uint32_t { SEG_VEC = 0, SEG_TRANS = 1, SEG_CONJ = 2 };
-struct seg_entry {
-    uint64_t s, d;
-    uint64_t lds_b, ldd_b;
-    uint32_t colb;
-    uint32_t vpc;
-    uint32_t slot0;
-    uint32_t vmul;  // division magic for vpc
-    uint32_t kind;
-    uint32_t ncols;  // element segments: final columns of the piece
-};
-constexpr int kMaxSeg = 64;
-// Transposed pieces whose final columns hold fewer bytes than this (counting
-// at most 8 bytes an element: f32 below 128 rows, the other types below 64),
-// or whose final column ranges are narrower (bytes), go to the element queue
-// instead of LDS tiles.
-#ifndef G2G_TRANS_ELEM_ROWB
-#define G2G_TRANS_ELEM_ROWB 512
-#endif
-#ifndef G2G_TRANS_ELEM_COLB
-#define G2G_TRANS_ELEM_COLB 256
-#endif
-constexpr uint32_t kTransElemRowBytes = G2G_TRANS_ELEM_ROWB;
-constexpr uint32_t kTransElemColBytes = G2G_TRANS_ELEM_COLB;
-
-__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t d, uint32_t mul) {
-    const uint32_t l = ceil_log2(d);
-    const uint32_t t = __umulhi(mul, n);
-    return l ? (t + ((n - t) >> 1)) >> (l - 1) : n;
-}
-
 // Copy every slot of the queued SEG_VEC segments. Slots are dealt to the
 // threads in order, so consecutive lanes move consecutive 16-byte vectors of a
 // column (a wave spans several short columns, or one stretch of a long one,
@@ -452,16 +404,10 @@ __device__ __forceinline__ void flat_copy(const seg_entry *segs, uint32_t total)
                     } while (next0 <= k);
                     e = segs[j];
                 }
-                const uint32_t rel = k - e.slot0;
-                const uint32_t c = magic_div(rel, e.vpc, e.vmul);
-                const uint32_t v = rel - c * e.vpc;
-                // column byte offset of the slot (v*16 < colb: no wrap)
-                uint32_t off = v * 16;
-                if (e.colb >= 16 && off > e.colb - 16) off = e.colb - 16;
-                const gbyte *sp = (const gbyte *)e.s + c * e.lds_b + off;
-                dp[u] = (gbyte *)e.d + c * e.ldd_b + off;
-                // dwords in this slot: 4, or 1-3 in a column shorter than 16 bytes
-                tail[u] = e.colb < 16 ? e.colb >> 2 : 4;
+                uint64_t so, dof;
+                vec_slot(e, k - e.slot0, so, dof, tail[u]);
+                const gbyte *sp = (const gbyte *)e.s + so;
+                dp[u] = (gbyte *)e.d + dof;
                 if (tail[u] == 4) {
                     r[u] = ld_a4<NT>(sp);
                 } else {
@@ -513,12 +459,8 @@ __device__ __forceinline__ void flat_copy_tiles(const seg_entry *segs, uint32_t 
             } while (next0 <= k);
             e = segs[j];
         }
-        const uint32_t rel = k - e.slot0;
-        const uint32_t g = magic_div(rel, e.vpc, e.vmul);
-        const uint32_t h = rel - g * e.vpc;
-        const uint32_t c = g * VE, i0 = h * VE;
-        const uint32_t nc = min(VE, e.ncols - c);               // columns of the group
-        const uint32_t nr = min(VE, e.colb / (uint32_t)ES - i0);  // rows of the group
+        uint32_t c, i0, nc, nr;  // nc columns from c, nr rows from i0
+        tile_slot<ES>(e, k - e.slot0, c, i0, nc, nr);
         const bool conj = ES > 4 && e.kind == SEG_CONJ;
         const gbyte *sp = (const gbyte *)e.s + (uint64_t)i0 * e.lds_b + (uint64_t)c * ES;
         u32x4 in[VE];
@@ -592,11 +534,8 @@ __device__ __forceinline__ void flat_copy_elems(const seg_entry *segs, uint32_t 
                     } while (next0 <= k);
                     e = segs[j];
                 }
-                const uint32_t rel = k - e.slot0;
-                const uint32_t g = magic_div(rel, e.vpc, e.vmul);
-                const uint32_t v = rel - g * e.vpc;
-                const uint32_t c = g * VE;
-                cnt[u] = min(VE, e.ncols - c);
+                uint32_t c, v;  // final row v of cnt final columns from c
+                elem_slot<ES>(e, k - e.slot0, c, v, cnt[u]);
                 dp[u] = (gbyte *)e.d + c * e.ldd_b + (uint64_t)v * ES;
                 ldd[u] = e.ldd_b;
                 conj[u] = ES > 4 && e.kind == SEG_CONJ;
@@ -668,20 +607,6 @@ __device__ __forceinline__ void copy_scaled(const piece &pc, const request &rq, 
     }
 }
 
-constexpr uint32_t kTileBytes = 16384;  // data bytes of one transpose tile
-constexpr uint32_t kSmemBytes = kTileBytes + 64 * 16;  // plus one padding element per tile row
-
-// Transpose tiles moved per pass of the full-tile path: enough that every
-// thread keeps 128 bytes of loads in flight (a transposing workgroup of 512
-// threads shares a CU with only one other, so one 16 KiB tile per pass left a
-// CU with 32 KiB of loads in flight). The pass's LDS (4 x 17 KiB for 512
-// threads, 2 x 17 KiB for 256) still lets 2 (512) or 3 (256) workgroups share
-// a CU.
-template <int THREADS>
-__host__ __device__ constexpr uint32_t trans_tiles() {
-    return THREADS >= 512 ? 4u : 2u;
-}
-
 // Transposing copy: destination(i, j) = op(source(j, i)) for final rows
 // i < rows and final columns j < ncols. Source element (i, j) lies at
 // s + (i*lds + j)*ES: storage rows run along final columns. A tile is TC final
@@ -698,15 +623,8 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
                                                 uint64_t ldd_b, uint32_t rows, uint32_t ncols,
                                                 uint32_t flags, LOCAL unsigned char *smem) {
     using T = typename bits_t<ES>::type;
-    constexpr uint32_t VE = 16 / ES;                     // elements per vector
-    constexpr uint32_t TR = kTileBytes / (TC * ES);      // final columns per tile
-    constexpr uint32_t LDW = TR + 1;                     // LDS row length in elements
-    constexpr uint32_t RV = TR / VE;                     // vectors per source run
-    constexpr uint32_t RVL = __builtin_ctz(RV);
-    constexpr uint32_t W = TC * ES / 16;                 // vectors per final column of a tile
-    constexpr uint32_t WL = __builtin_ctz(W);
-    constexpr uint32_t PER = kTileBytes / 16 / THREADS;  // vectors per thread per phase
-    static_assert(TC * LDW * ES <= kSmemBytes, "transpose tile exceeds its LDS");
+    using TG = tile_geom<ES, TC, THREADS>;
+    constexpr uint32_t VE = TG::VE, TR = TG::TR, LDW = TG::LDW, PER = TG::PER;
     LOCAL T *tile = (LOCAL T *)smem;               // tile[c * LDW + r]
     const bool conj = (flags & REQ_CONJ) != 0;
     const uint32_t n_ic = (rows + TC - 1) / TC;
@@ -719,8 +637,8 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
         const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
 #pragma unroll
         for (uint32_t k = 0; k < PER; ++k) {
-            const uint32_t q = threadIdx.x + k * THREADS;
-            const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+            uint32_t c, r0;
+            TG::load_index(threadIdx.x + k * THREADS, c, r0);
             const uint32_t i = ic + c, j0 = jc + r0;
             cnt1[k] = (i < rows && j0 < ncols) ? min(VE, ncols - j0) : 0u;
             const gbyte *p = s + i * lds_b + (uint64_t)j0 * ES;
@@ -738,8 +656,8 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
         const uint32_t jc = (t / n_ic) * TR, ic = (t % n_ic) * TC;
 #pragma unroll
         for (uint32_t k = 0; k < PER; ++k) {
-            const uint32_t q = threadIdx.x + k * THREADS;
-            const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+            uint32_t c, r0;
+            TG::load_index(threadIdx.x + k * THREADS, c, r0);
             LOCAL T *tp = tile + c * LDW + r0;
 #pragma unroll
             for (uint32_t e = 0; e < VE; ++e)
@@ -749,8 +667,8 @@ __device__ __forceinline__ void copy_transposed(const gbyte *__restrict__ s,
         if (t + 1 < n_tiles) load_tile(t + 1);
 #pragma unroll
         for (uint32_t k = 0; k < PER; ++k) {
-            const uint32_t q = threadIdx.x + k * THREADS;
-            const uint32_t r = q >> WL, w = q & (W - 1);
+            uint32_t r, w;
+            TG::store_index(threadIdx.x + k * THREADS, r, w);
             const uint32_t j = jc + r, i0 = ic + w * VE;
             if (j < ncols && i0 < rows) {
                 gbyte *p = d + j * ldd_b + (uint64_t)i0 * ES;
@@ -803,17 +721,10 @@ __device__ __forceinline__ void copy_transposed_full(const gbyte *__restrict__ s
                                                      uint64_t ldd_in, uint32_t n_ic_in, uint32_t n_tiles_in,
                                                      uint32_t flags, LOCAL unsigned char *smem) {
     using T = typename bits_t<ES>::type;
+    using TG = tile_geom<ES, TC, THREADS>;
     constexpr uint32_t G = trans_tiles<THREADS>();
-    constexpr uint32_t VE = 16 / ES;
-    constexpr uint32_t TR = kTileBytes / (TC * ES);
-    constexpr uint32_t LDW = TR + 1;
-    constexpr uint32_t RV = TR / VE;
-    constexpr uint32_t RVL = __builtin_ctz(RV);
-    constexpr uint32_t W = TC * ES / 16;
-    constexpr uint32_t WL = __builtin_ctz(W);
-    constexpr uint32_t PER = kTileBytes / 16 / THREADS;
-    constexpr uint32_t TILE_ELEMS = kSmemBytes / ES;     // LDS area of one tile, in elements
-    static_assert(TC * LDW * ES <= kSmemBytes, "transpose tile exceeds its LDS");
+    constexpr uint32_t VE = TG::VE, TR = TG::TR, LDW = TG::LDW, PER = TG::PER;
+    constexpr uint32_t TILE_ELEMS = TG::TILE_ELEMS;  // LDS area of one tile, in elements
     const gbyte *s = (const gbyte *)uniform64((uint64_t)s_in);
     gbyte *d = (gbyte *)uniform64((uint64_t)d_in);
     const uint64_t lds_b = uniform64(lds_in), ldd_b = uniform64(ldd_in);
@@ -825,11 +736,11 @@ __device__ __forceinline__ void copy_transposed_full(const gbyte *__restrict__ s
     uint32_t so[PER], dof[PER], lw[PER], lr[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t q = threadIdx.x + k * THREADS;
-        const uint32_t c = q >> RVL, r0 = (q & (RV - 1)) * VE;
+        uint32_t c, r0, r, w;
+        TG::load_index(threadIdx.x + k * THREADS, c, r0);
         so[k] = c * (uint32_t)lds_b + r0 * ES;
         lw[k] = c * LDW + r0;
-        const uint32_t r = q >> WL, w = q & (W - 1);
+        TG::store_index(threadIdx.x + k * THREADS, r, w);
         dof[k] = r * (uint32_t)ldd_b + w * VE * ES;
         lr[k] = (w * VE) * LDW + r;
     }
@@ -974,25 +885,6 @@ __device__ __forceinline__ void process_piece(const piece &pc, const request &rq
         transpose_call<THREADS, NT, 16>(s, d, lds_b, ldd_b, pc.rows, ncols, rq.flags, smem);
 }
 
-// Columns [c0, c1) of a piece that belong to the work item covering stream
-// bytes [lo, hi), for a piece occupying stream bytes [ps, pe): the columns
-// (groups of 256/es columns for transposed pieces) whose first byte lies in
-// [lo, hi).
-__device__ __forceinline__ void piece_cols(uint32_t rows, uint32_t cols, uint32_t flags, uint32_t es,
-                                           uint64_t ps, uint64_t pe, uint64_t lo, uint64_t hi,
-                                           uint32_t &c0, uint32_t &c1) {
-    c0 = 0;
-    c1 = cols;
-    if (ps < lo || pe > hi) {  // piece shared with neighbouring work items
-        const uint64_t q = (flags & REQ_TRANS) ? 256u / es : 1u;
-        const uint64_t qb = (uint64_t)rows * es * q;
-        const uint64_t a0 = lo > ps ? ((lo - ps + qb - 1) / qb) * q : 0;
-        const uint64_t a1 = ((hi - ps + qb - 1) / qb) * q;
-        c0 = (uint32_t)min(a0, (uint64_t)cols);
-        c1 = (uint32_t)min(a1, (uint64_t)cols);
-    }
-}
-
 __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t lane) {
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -1001,8 +893,6 @@ __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t lane) {
     }
     return x;
 }
-
-enum : uint32_t { PK_NONE = 0, PK_VEC = 1, PK_ELEM = 2, PK_DIRECT = 3 };
 
 // Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
 // stream (pieces concatenated in table order, each column-major). It moves
@@ -1042,26 +932,9 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
             flags = (pc.req & PIECE_REQ1) ? kr.r[1].flags : kr.r[0].flags;
             rows = pc.rows;
             piece_cols(rows, pc.cols, flags, es, ps, pe, lo, hi, c0, c1);
-            if (in && c0 < c1) {
-                const bool trans = (flags & REQ_TRANS) != 0;
-                if (FEAT == FEAT_COPY) {
-                    // the copy-only variant is launched for untransposed,
-                    // unscaled requests only; anything else is left alone
-                    kind = (flags & (REQ_TRANS | REQ_SCALE)) ? PK_NONE : PK_VEC;
-                } else if (flags & REQ_SCALE) {
-                    kind = PK_DIRECT;
-                } else if (trans) {
-                    // transposed pieces that would leave an LDS tile mostly
-                    // empty go to the element queue (register tiles)
-                    kind = ((uint64_t)rows * min(es, 8u) < kTransElemRowBytes ||
-                            (uint64_t)(c1 - c0) * es < kTransElemColBytes)
-                               ? PK_ELEM : PK_DIRECT;
-                } else {
-                    kind = PK_VEC;
-                }
-                // copy slots: 16-byte vectors per column; element slots: groups
-                // of 16/es final rows (register tiles), or rows for 16-byte types
-                vpc = kind == PK_VEC ? (rows * es + 15) >> 4 : es < 16 ? (rows * es + 15) >> 4 : rows;
+            if (in) {
+                kind = piece_kind(FEAT, flags, rows, c0, c1, es);
+                vpc = piece_vpc(kind, rows, es);
             }
         }
         // pieces past the work item come after every piece inside it
@@ -1072,27 +945,15 @@ __device__ __forceinline__ void process_tile(const work_rec &job, const kreqs &k
         uint64_t md = __ballot(kind == PK_DIRECT);
         const uint32_t nseg = (uint32_t)__popcll(mv), ntseg = (uint32_t)__popcll(me);
         if (nseg | ntseg) {
-            // element segments: one slot per final row and group of 16/es columns
-            const uint32_t groups = is_e ? (c1 - c0 + (16u / es) - 1) / (16u / es) : c1 - c0;
-            const uint32_t slots = (is_v || is_e) ? groups * vpc : 0u;
+            const uint32_t slots = (is_v || is_e) ? piece_groups(kind, c1 - c0, es) * vpc : 0u;
             const uint32_t sv = wave_scan_u32(is_v ? slots : 0u, lane);
             const uint32_t se = wave_scan_u32(is_e ? slots : 0u, lane);
             const uint32_t total = __shfl(sv, 63, 64), ttotal = __shfl(se, 63, 64);
             if (threadIdx.x < 64) {
                 const uint64_t below = (1ull << lane) - 1;
                 if (is_v || is_e) {
-                    const uint64_t ldd_b = (uint64_t)pc.ldd * es, lds_b = (uint64_t)pc.lds * es;
                     seg_entry &e = is_v ? segs[__popcll(mv & below)] : tsegs[__popcll(me & below)];
-                    e.s = pc.src + (uint64_t)c0 * (is_v ? lds_b : es);
-                    e.d = pc.dst + (uint64_t)c0 * ldd_b;
-                    e.lds_b = lds_b;
-                    e.ldd_b = ldd_b;
-                    e.colb = rows * es;
-                    e.vpc = vpc;
-                    e.slot0 = is_v ? sv - slots : se - slots;
-                    e.vmul = is_v ? pc.vmul : div_magic(vpc);
-                    e.kind = is_v ? SEG_VEC : (flags & REQ_CONJ) ? SEG_CONJ : SEG_TRANS;
-                    e.ncols = c1 - c0;
+                    e = make_seg(pc, kind, flags, es, c0, c1 - c0, vpc, is_v ? sv - slots : se - slots);
                 }
                 if (lane == 0) {  // sentinels
                     segs[nseg].slot0 = total;

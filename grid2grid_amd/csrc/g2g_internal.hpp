@@ -8,6 +8,7 @@
 #include <vector>
 
 struct g2g_plan_s;
+struct g2g_layout;  // include/grid2grid_amd.h
 
 namespace g2g {
 

@@ -68,37 +68,9 @@ __global__ void k_plan_pieces(const span *urows, int32_t n_ur, const span *ucols
     const span R = urows[ui], C = ucols[uj];
     const cell F = fblk[(int64_t)R.fblk * n_fc + C.fblk];
     const cell I = iblk[(int64_t)R.iblk * n_ic + C.iblk];
-    const uint32_t rows = (uint32_t)(R.end - R.start), cols = (uint32_t)(C.end - C.start);
-    const uint64_t dst =
-        F.ptr + ((uint64_t)(C.start - fcs[C.fblk]) * F.ld + (uint64_t)(R.start - frs[R.fblk])) * es;
-    uint64_t src;
-    if (!trans)
-        src = I.ptr + ((uint64_t)(C.start - ics[C.iblk]) * I.ld + (uint64_t)(R.start - irs[R.iblk])) * es;
-    else
-        src = I.ptr + ((uint64_t)(R.start - irs[R.iblk]) * I.ld + (uint64_t)(C.start - ics[C.iblk])) * es;
-    piece pc;
-    pc.src = src;
-    pc.dst = dst;
-    pc.lds = (uint32_t)I.ld;
-    pc.ldd = (uint32_t)F.ld;
-    pc.rows = rows;
-    pc.cols = cols;
-    // the reference call that would scale this piece: a local transposed
-    // piece goes through copy_transpose_and_scale (communication_data.cpp:
-    // 197-210); every other piece through copy2D_and_scale, as one loop when
-    // the piece is contiguous on both sides (the packed receive buffer has
-    // ld = rows; a local piece reads the initial block) and column by column
-    // otherwise (memory_utils.hpp:79-122)
-    const bool local = I.owner == F.owner;
-    uint32_t site;
-    if (local && trans)
-        site = PIECE_SITE_TRANS;
-    else
-        site = (F.ld == rows && (!local || I.ld == rows)) ? PIECE_SITE_FLAT : PIECE_SITE_COL;
-    pc.req = site << PIECE_SITE_SHIFT;
-    pc.vmul = div_magic((rows * es + 15) / 16);
+    const piece pc = make_piece(R, C, F, I, frs[R.fblk], fcs[C.fblk], irs[R.iblk], ics[C.iblk], es, trans != 0);
     pieces[p] = pc;
-    bytes[p] = (uint64_t)rows * cols * es;
+    bytes[p] = (uint64_t)pc.rows * pc.cols * es;
     ranks[p] = ((uint32_t)I.owner << 16) | (uint32_t)F.owner;
 }
 
@@ -127,14 +99,9 @@ __global__ void k_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
     if (k >= n_chunks * n_ur) return;
     const int64_t g = k / n_ur;
     const int32_t ui = (int32_t)(k - g * n_ur);
-    const uint32_t uj = chunk_uj[g], c0 = chunk_c0[g];
-    piece pc = pieces[(int64_t)uj * n_ur + ui];
-    const uint32_t nc = min(w, pc.cols - c0);
-    pc.src += trans ? (uint64_t)c0 * es : (uint64_t)c0 * pc.lds * es;
-    pc.dst += (uint64_t)c0 * pc.ldd * es;
-    pc.cols = nc;
+    const piece pc = chunk_piece(pieces[(int64_t)chunk_uj[g] * n_ur + ui], chunk_c0[g], w, es, trans != 0);
     out[k] = pc;
-    bytes[k] = (uint64_t)pc.rows * nc * es;
+    bytes[k] = (uint64_t)pc.rows * pc.cols * es;
 }
 
 hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t *chunk_uj,

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+import shapes
 import grid2grid_amd as g2g
 from grid2grid_amd.layout import scalapack_layout, arbitrary_layout
 
@@ -29,42 +30,11 @@ def _rand_arena(layout, dtype, rng):
 
 
 def _layout_pairs():
-    rng = np.random.default_rng(101)
-    return {
-        "bc_small": (scalapack_layout(37, 29, 4, 3, 2, 2, "C"), scalapack_layout(37, 29, 5, 7, 2, 2, "C")),
-        "bc_grid_change": (scalapack_layout(100, 77, 8, 8, 2, 3, "R"), scalapack_layout(100, 77, 3, 5, 3, 1, "C")),
-        "bc_4096_blocks": (scalapack_layout(256, 256, 32, 32, 2, 2, "C", lld="rows"),
-                          scalapack_layout(256, 256, 64, 128, 2, 2, "C", lld="rows")),
-        "arbitrary": (arbitrary_layout(rng, 300, 260, 7, 6, 5, lo=10, hi=80, pad=(0, 9), gap=(0, 33)),
-                      arbitrary_layout(rng, 300, 260, 5, 8, 6, lo=10, hi=80, pad=(0, 9), gap=(0, 33))),
-        "single_rank": (scalapack_layout(64, 48, 64, 48, 1, 1), scalapack_layout(64, 48, 16, 16, 1, 1)),
-        "tall_thin": (scalapack_layout(1000, 3, 17, 1, 4, 1), scalapack_layout(1000, 3, 300, 2, 2, 2)),
-        # more ranks than the LDS pair table holds: the plain form's wire
-        # order comes from a radix sort by rank pair
-        "many_ranks": (arbitrary_layout(rng, 240, 180, 16, 12, 130, lo=5, hi=25, pad=(0, 5), gap=(0, 9)),
-                       arbitrary_layout(rng, 240, 180, 13, 15, 100, lo=5, hi=25, pad=(0, 5), gap=(0, 9))),
-    }
+    return shapes.layout_pairs()
 
 
 def _transposed_pairs():
-    rng = np.random.default_rng(202)
-    return {
-        "bcT": (scalapack_layout(70, 93, 8, 5, 2, 2, "C"), scalapack_layout(93, 70, 6, 9, 2, 2, "C")),
-        "arbT": (arbitrary_layout(rng, 130, 210, 4, 5, 4, lo=10, hi=80, pad=(0, 6), gap=(0, 17)),
-                 arbitrary_layout(rng, 210, 130, 6, 3, 3, lo=10, hi=80, pad=(0, 6), gap=(0, 17))),
-        "bcT_big_tiles": (scalapack_layout(160, 96, 64, 48, 2, 1, "C"), scalapack_layout(96, 160, 40, 64, 1, 2, "C")),
-        # pieces of 100 and 60 final rows by 128 and 44 final columns: both the
-        # LDS tile path and the element path of transposed pieces
-        "bcT_mixed_paths": (scalapack_layout(300, 260, 128, 100, 2, 2, "C"),
-                            scalapack_layout(260, 300, 100, 128, 2, 2, "C")),
-        # pieces of 8-24 final rows by up to 96 final columns: 16-row tiles
-        # and the element queue
-        "bcT_short_tiles": (scalapack_layout(400, 300, 96, 40, 2, 2, "C"),
-                            scalapack_layout(300, 400, 24, 160, 2, 2, "C")),
-        # pieces of 16-40 final rows by up to 200 final columns: 32- and 64-row tiles
-        "bcT_mid_tiles": (scalapack_layout(400, 300, 200, 56, 2, 2, "C"),
-                          scalapack_layout(300, 400, 40, 256, 2, 2, "C")),
-    }
+    return shapes.transposed_pairs()
 
 
 def _expected(init, ia, fin, fa, dtype, trans="N", alpha=None, beta=None):
@@ -258,9 +228,7 @@ def test_ragged_column_tails(dtype):
     than 16 bytes dword by dword. Every byte of the final arena (padding and
     gaps included) matches the oracle, for every work-item size, launch
     mode and the column-chunked stream."""
-    rng = np.random.default_rng(515 + dtype)
-    init = arbitrary_layout(rng, 150, 70, 27, 5, 3, lo=1, hi=11, pad=(0, 3), gap=(0, 5))
-    fin = arbitrary_layout(rng, 150, 70, 23, 6, 4, lo=1, hi=11, pad=(0, 3), gap=(0, 5))
+    init, fin, rng = shapes.ragged_pair(dtype)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
     want = _expected(init, ia, fin, fa, dtype)
     ti = _dev(ia)
@@ -317,7 +285,7 @@ def test_column_chunked_stream(name):
     """The gather over column chunks (pieces cut into chunks of w columns,
     ordered down each column interval) writes exactly what the pieces in
     their own order write, for odd widths and every launch mode; the width is
-    reported back; transposed plans refuse it."""
+    reported back; a negative width is refused."""
     init, fin = _layout_pairs()[name]
     dtype = g2g.FLOAT32
     rng = np.random.default_rng(17)
@@ -350,7 +318,7 @@ def test_column_chunked_stream(name):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("name", ["bcT", "arbT", "bcT_mixed_paths", "bcT_short_tiles", "bcT_mid_tiles"])
+@pytest.mark.parametrize("name", ["bcT", "arbT", "bcT_mixed_paths", "bcT_short_tiles", "bcT_mid_tiles", "bcT_cut_edges"])
 def test_column_chunked_transposed(name, dtype):
     """Transposed pieces cut into chunks of final columns (source offsets
     along the storage rows): whole LDS tiles, strips and the element queue

@@ -102,6 +102,8 @@ PROTOTYPES = {
     "g2g_batch_execute_graph": (_i, [_vp, _vp]),
     "g2g_batch_bytes": (_i, [_vp, _p64]),
     "g2g_batch_autotune": (_i, [_vp, _vp, _i, ctypes.POINTER(G2GTuning)]),
+    "g2g_batch_get_tuning": (_i, [_vp, ctypes.POINTER(G2GTuning)]),
+    "g2g_batch_set_tuning": (_i, [_vp, ctypes.POINTER(G2GTuning), _vp]),
     "g2g_batch_destroy": (_i, [_vp]),
     "g2g_transform_plans_noalloc": (_i, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "g2g_set_tuning": (_i, [_i64, _i]),

@@ -182,6 +182,17 @@ class Batch:
         check(_lib.load().g2g_batch_autotune(self.handle, _stream(stream), reps, ctypes.byref(t)))
         return _tuning_dict(t)
 
+    @property
+    def tuning(self):
+        t = _lib.G2GTuning()
+        check(_lib.load().g2g_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, tile_bytes, threads=512, unroll_bytes=128, nontemporal=0, launch_mode=1, stream=None):
+        """launch_mode 1 (one work counter) or 3 (one per XCD); batches always launch persistent."""
+        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode), 0, 0, 0.0)
+        check(_lib.load().g2g_batch_set_tuning(self.handle, ctypes.byref(t), _stream(stream)))
+
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().g2g_batch_destroy(self.handle)

@@ -1455,6 +1455,35 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
     return G2G_OK;
 }
 
+int g2g_batch_get_tuning(g2g_batch b, g2g_tuning *t) {
+    if (!b || !t) return fail(G2G_ERR_INVALID, "NULL argument");
+    t->tile_bytes = (int64_t)b->tile_bytes;
+    t->threads = b->cfg.threads;
+    t->unroll_bytes = b->cfg.unroll_bytes;
+    t->nontemporal = b->cfg.nontemporal;
+    t->launch_mode = launch_mode_of(b->cfg);
+    t->column_chunks = 0;
+    t->reserved = 0;
+    t->ms = b->tuned_ms;
+    return G2G_OK;
+}
+
+int g2g_batch_set_tuning(g2g_batch b, const g2g_tuning *t, void *stream) {
+    if (!b || !t) return fail(G2G_ERR_INVALID, "NULL argument");
+    if (t->tile_bytes < 4096 || t->tile_bytes > (1ll << 26) || (t->threads != 256 && t->threads != 512) ||
+        (t->unroll_bytes != 64 && t->unroll_bytes != 128) ||
+        (t->launch_mode != G2G_LAUNCH_PERSISTENT && t->launch_mode != G2G_LAUNCH_PERSISTENT_XCD))
+        return fail(G2G_ERR_INVALID, "batch tuning outside the compiled variants (batches launch persistent)");
+    if (t->column_chunks != 0) return fail(G2G_ERR_INVALID, "a batch runs each plan's own stream: column_chunks must be 0");
+    if ((uint64_t)t->tile_bytes != b->tile_bytes) G2G_TRY(batch_retile(b, (uint64_t)t->tile_bytes, (hipStream_t)stream));
+    b->drop_graph();
+    b->cfg.threads = t->threads;
+    b->cfg.unroll_bytes = t->unroll_bytes;
+    b->cfg.nontemporal = t->nontemporal != 0;
+    set_launch_mode(b->cfg, t->launch_mode);
+    return G2G_OK;
+}
+
 int g2g_batch_bytes(g2g_batch b, int64_t *bytes) {
     if (!b || !bytes) return fail(G2G_ERR_INVALID, "NULL argument");
     *bytes = (int64_t)b->total_bytes;

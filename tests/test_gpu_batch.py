@@ -100,6 +100,32 @@ def test_batch_launch_modes_and_autotune(launch_mode):
     batch.close()
 
 
+def test_batch_set_tuning_every_variant():
+    """Every settable batch configuration (work-item size, workgroup size,
+    bytes in flight, non-temporal hint, both persistent modes) gives every
+    request's own result; settings outside the compiled variants are refused."""
+    cases = _cases(12, 9, scaled_every=4)
+    batch = g2g.Batch([c.plan for c in cases], [c.alpha for c in cases], [c.beta for c in cases])
+    for tb, th, ub, nt, mode in [(4096, 256, 64, 0, 1), (32768, 512, 128, 1, 3), (1 << 20, 512, 64, 0, 3),
+                                 (65536, 256, 128, 1, 1)]:
+        batch.set_tuning(tb, th, ub, nt, mode)
+        t = batch.tuning
+        assert (t["tile_bytes"], t["threads"], t["unroll_bytes"], t["nontemporal"], t["launch_mode"]) == \
+            (tb, th, ub, nt, mode)
+        for c in cases:
+            c.reset()
+        torch.cuda.synchronize()
+        batch.execute()
+        torch.cuda.synchronize()
+        for k, c in enumerate(cases):
+            c.check(f"tuning {t} request {k}")
+    for bad in [dict(tile_bytes=1024), dict(tile_bytes=32768, threads=128), dict(tile_bytes=32768, launch_mode=0),
+                dict(tile_bytes=32768, unroll_bytes=96)]:
+        with pytest.raises(g2g.G2GError):
+            batch.set_tuning(**bad)
+    batch.close()
+
+
 def test_batch_graph_replays():
     """The captured graph gives the eager result on every replay, with eager
     work and synchronizations between the replays."""

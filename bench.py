@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=6.0, help="budget of the CPU baseline's 8192^2 sample")
     ap.add_argument("--batch-requests", type=int, default=2048)
     ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
     ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
@@ -123,24 +123,144 @@ def read_traffic(workload, mode):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(budget_s):
-    """The reference's own pipeline (oracle/_ref, compiled from the reference
-    sources) on a bounded sample of the default workload: 8192^2 f64 with the
-    same block sizes and rank grids (128x128 on 4x4 -> 256x512 on 2x8), 16
-    virtual ranks in one process, OpenMP over all host threads. Falls back to
-    the C restatement (single thread) when the reference build is absent."""
+def roofline(nbytes, per_launch_ms, workload, mode="fused"):
+    achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(workload, mode),
+            "algorithmic_bytes": int(nbytes)}
+
+
+def launch_stats(per):
+    return {"mean": round(statistics.mean(per), 4), "median": round(statistics.median(per), 4),
+            "min": round(min(per), 4), "max": round(max(per), 4), "launches": len(per)}
+
+
+def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate=1, facade_calls=0):
+    """One workload measured on its own: `rotate` buffer pairs, each with its
+    own plan, run round-robin (one step = one transform of one pair), so that
+    a working set below the 256 MB Infinity Cache is still read from HBM.
+    The first plan is autotuned; the others start from the tuning it records
+    for their shape. With facade_calls, the drop-in call (transform_layouts)
+    is timed on the same pairs too."""
+    init, fin, dtype, trans = build_workload(name)
+    es = g2g.DTYPE_SIZE[dtype]
+    pairs = []
+    t0 = time.time()
+    for k in range(rotate):
+        ti = torch.empty(init.arena_elems * es, dtype=torch.uint8, device=dev)
+        tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device=dev)
+        g2g.fill_layout(init, ti.data_ptr(), dtype, 100 + k, stream)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans, stream)
+        if k == 0:
+            tuning = plan.autotune(stream=stream, reps=3)
+        pairs.append((ti, tf, plan))
+    later = pairs[-1][2].tuning
+    log(f"leg {name}: {rotate} buffer pair(s), planned and tuned in {time.time() - t0:.1f}s: {tuning}")
+    it = [0]
+
+    def step():
+        pairs[it[0] % rotate][2].transform(stream=stream)
+        it[0] += 1
+
+    n = max(steps, rotate)
+    n -= n % rotate  # every pair the same number of times
+    ms, per = time_steps(torch, step, n, max(warmup, rotate), stream, per_launch=True)
+    bad = sum(g2g.verify_layout(fin, tf.data_ptr(), dtype, 100 + k, trans, init.n, stream)
+              for k, (_, tf, _) in enumerate(pairs))
+    nbytes = workloads.algorithmic_bytes(fin, dtype)
+    info = pairs[0][2].info
+    out = {
+        "workload": WORKLOAD_DESC.get(name, name),
+        "dtype": {g2g.FLOAT32: "f32", g2g.FLOAT64: "f64", g2g.COMPLEX64: "c64", g2g.COMPLEX128: "c128"}[dtype],
+        "buffer_pairs": rotate,
+        "working_set_bytes": int(rotate * (init.arena_elems + fin.arena_elems) * es),
+        "steps": n,
+        "ms_per_step": round(ms, 4),
+        "value": round(nbytes / (ms * 1e-3) / 1e9, 2),
+        "unit": "GB/s",
+        "bit_exact": bad == 0,
+        "mismatches": int(bad),
+        "pieces": int(info.n_pieces),
+        "work_items": int(info.n_tiles),
+        "tuning": {k: tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes", "nontemporal", "launch_mode",
+                                          "column_chunks")},
+        "later_pairs_start_tuned": all(later[k] == tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes",
+                                                                         "nontemporal", "launch_mode")),
+        "launch_ms": launch_stats(per),
+        "roofline": roofline(nbytes, statistics.mean(per), name),
+    }
+    log(f"leg {name}: {ms:.4f} ms/step, {out['value']} GB/s, frac {out['roofline']['frac']}, mismatches {bad}")
+    if facade_calls:
+        out["facade"] = facade_leg(torch, g2g, name, init, fin, dtype, trans, pairs,
+                                   [100 + k for k in range(rotate)], stream, facade_calls)
+    for _, _, plan in pairs:
+        plan.close()
+    del pairs
+    torch.cuda.empty_cache()
+    return out
+
+
+def facade_leg(torch, g2g, name, init, fin, dtype, trans, pairs, seeds, stream, calls):
+    """The drop-in call g2g_transform_layouts (what the C++ facade's
+    transform() runs), one call per step over the rotating buffer pairs,
+    each call waited for as the reference's transform() returns after its
+    exchange: with the plan cache off (every call plans, launches and frees
+    its plan, as the reference plans on every call) and on (every call after
+    the first per pair replays the cached plan). Host wall time per call."""
+    def call(k):
+        ti, tf, _ = pairs[k % len(pairs)]
+        g2g.transform_layouts(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans, stream=stream)
+        stream.synchronize()
+
+    def timed(n):
+        ts = []
+        for k in range(n):
+            t0 = time.perf_counter()
+            call(k)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return ts
+
+    out = {}
+    for tf in (p[1] for p in pairs):
+        tf.zero_()
+    g2g.plan_cache_clear()
+    g2g.plan_cache_set_capacity(0)
+    try:
+        call(0)  # first-call costs of the process (module load, pools) outside the sample
+        unc = timed(calls)
+    finally:
+        g2g.plan_cache_set_capacity(64)
+    g2g.plan_cache_clear()
+    for k in range(len(pairs)):
+        call(k)
+    hits0, misses0, _ = g2g.plan_cache_stats()
+    cac = timed(calls)
+    hits, misses, entries = g2g.plan_cache_stats()
+    bad = sum(g2g.verify_layout(fin, p[1].data_ptr(), dtype, seed, trans, init.n, stream)
+              for p, seed in zip(pairs, seeds))
+    g2g.plan_cache_clear()
+    for key, ts in (("uncached", unc), ("cached", cac)):
+        out[key] = {"ms_per_call_median": round(statistics.median(ts), 4), "ms_per_call_mean": round(statistics.mean(ts), 4),
+                    "calls": len(ts)}
+    out["cache_hits_in_sample"] = int(hits - hits0)
+    out["cache_misses_in_sample"] = int(misses - misses0)
+    out["uncached_over_cached"] = round(out["uncached"]["ms_per_call_median"] / out["cached"]["ms_per_call_median"], 2)
+    out["bit_exact"] = bad == 0
+    log(f"facade {name}: {out}")
+    return out
+
+
+def _cpu_sample(O, m, warmups, runs, budget_s, kind):
+    """Median seconds of the CPU transform of the default workload's blocks and
+    grids at size m x m: `warmups` untimed runs, then timed runs until both
+    `runs` runs and `budget_s` seconds are done (at most 200)."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
     from grid2grid_amd import workloads
-    init, fin, dtype, _ = workloads.bc32768(m=8192)
+    init, fin, dtype, _ = workloads.bc32768(m=m)
     es = O.ES[dtype]
     ia = np.zeros(init.arena_elems * es, dtype=np.uint8)
     fa = np.zeros(fin.arena_elems * es, dtype=np.uint8)
     ia[::4096] = 1
-    nbytes = 2 * init.m * init.n * es
-    kind = "reference" if O.ref() is not None else "port"
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if kind == "reference" else 1
     times = []
     t_end = time.time() + budget_s
     n = 0
@@ -152,29 +272,55 @@ def cpu_baseline(budget_s):
             O.transform(init, ia, fin, fa, dtype)
         dt = time.perf_counter() - t0
         n += 1
-        if n > 2:  # two warm-ups
+        if n > warmups:
             times.append(dt)
-        if (time.time() > t_end and len(times) >= 3) or len(times) >= 200:
+        if (time.time() > t_end and len(times) >= runs) or len(times) >= 200:
             break
-    med = statistics.median(times)
+    del ia, fa
+    return statistics.median(times), len(times), 2 * m * m * es
+
+
+def cpu_baseline(budget_s):
+    """The reference's own pipeline (oracle/_ref, compiled from the reference
+    sources) on the default workload itself, 32768^2 f64 (128x128 on 4x4 ->
+    256x512 on 2x8, 16 virtual ranks in one process, OpenMP over all host
+    threads; 16 GiB of host arrays): median of 5 runs after one warm-up; and,
+    as a second sample, 8192^2 with the same blocks and grids. Without the
+    reference build, the C restatement (single thread) on the 8192^2 sample
+    only."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    kind = "reference" if O.ref() is not None else "port"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if kind == "reference" else 1
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
             cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
     except OSError:
         pass
+    host = {"cpu_model": cpu_model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    med_s, n_s, bytes_s = _cpu_sample(O, 8192, 2, 3, budget_s, kind)
+    small = {"value": round(bytes_s / med_s / 1e9, 3), "unit": "GB/s",
+             "sample": f"8192^2 f64, same blocks/grids, median of {n_s} runs after 2 warm-ups",
+             "ms_per_transform": round(med_s * 1e3, 3)}
+    if kind != "reference":
+        return {"value": small["value"], "unit": "GB/s", "cores": 1, "kind": kind,
+                "sample": small["sample"] + " (C restatement, one thread); time includes planning",
+                "ms_per_transform": small["ms_per_transform"], "host": host}
+    med, n, nbytes = _cpu_sample(O, 32768, 1, 5, 0.0, kind)
     return {
         "value": round(nbytes / med / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"8192^2 f64, the default workload's blocks/grids (128x128 on 4x4 -> 256x512 on 2x8), 16 virtual ranks "
-                  f"in one process, median of {len(times)} runs after 2 warm-ups, {threads} OpenMP threads; "
-                  f"time includes the reference's per-call planning",
+        "sample": f"the full workload: 32768^2 f64, 128x128 on 4x4 -> 256x512 on 2x8, 16 virtual ranks in one "
+                  f"process, median of {n} runs after 1 warm-up, {threads} OpenMP threads; time includes the "
+                  f"reference's per-call planning",
         "ms_per_transform": round(med * 1e3, 3),
-        "host": {"cpu_model": cpu_model, "nproc": os.cpu_count(),
-                 "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
-                 "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")},
+        "second_sample": small,
+        "host": host,
     }
 
 
@@ -445,17 +591,8 @@ def main():
         "mismatches": int(bad),
         # every launch of the second pass timed on its own (HIP events on the
         # launch stream): the spread behind roofline.achieved, which uses the mean
-        "launch_ms": {"mean": round(per_launch_ms, 4), "median": round(statistics.median(per), 4),
-                      "min": round(min(per), 4), "max": round(max(per), 4), "launches": len(per)},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(args.workload, args.mode),
-            "algorithmic_bytes": int(nbytes),
-        },
+        "launch_ms": launch_stats(per),
+        "roofline": roofline(nbytes, per_launch_ms, args.workload, args.mode),
     }
 
     # streaming reference rates of this device on the same buffers (16-byte
@@ -480,11 +617,22 @@ def main():
         result["plain_staged"] = {"ms_per_step": round(ms_s, 4), "value": round(nbytes / (ms_s * 1e-3) / 1e9, 2),
                                   "unit": "GB/s", "bit_exact": bad_s == 0}
         log(f"staged: {ms_s:.3f} ms/step, mismatches {bad_s}")
+    if not args.no_extras and args.mode == "fused":
+        tf.zero_()
+        result["facade"] = facade_leg(torch, g2g, args.workload, init, fin, dtype, trans, [(ti, tf, plan)], [2],
+                                      stream, 10)
     plan.close()
     del ti, tf
     torch.cuda.empty_cache()
 
     if not args.no_extras:
+        # the other single-transform workloads, each with its own roofline;
+        # 4096^2 (2 x 128 MiB) rotates 8 buffer pairs (2 GiB) past the 256 MB Infinity Cache
+        result["legs"] = {}
+        for name, rotate, steps, calls in (("bc4096", 8, 400, 80), ("irregular", 1, args.steps, 0)):
+            if name != args.workload:
+                result["legs"][name] = workload_leg(torch, g2g, workloads, name, stream, dev, steps, args.warmup,
+                                                    rotate=rotate, facade_calls=calls)
         result["batched"] = batched_leg(torch, g2g, workloads, args.batch_requests, 5, stream, dev,
                                         args.batch_ptrans, args.batch_tile_bytes)
         log(f"batched: {result['batched']}")

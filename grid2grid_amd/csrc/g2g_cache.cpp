@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -52,38 +53,59 @@ void retire_plan(g2g_plan_s *p) {
 
 void put(std::string &k, const void *p, size_t n) { k.append((const char *)p, n); }
 
+// g2g_block has no padding: its bytes are exactly its four fields
+static_assert(sizeof(g2g_block) == 24 && offsetof(g2g_block, stride) == 8 && offsetof(g2g_block, data) == 16,
+              "g2g_block layout");
+
 // The descriptor's bytes, or false when its arrays are not even well formed
 // (g2g_plan_create then reports what is wrong).
 bool layout_key(const g2g_layout *l, std::string &k) {
     if (!l || l->n_rows_split < 1 || l->n_cols_split < 1 || !l->rows_split || !l->cols_split || !l->owners ||
         l->n_blocks < 0 || (l->n_blocks > 0 && !l->blocks))
         return false;
+    const size_t cells = (size_t)(l->n_rows_split - 1) * (size_t)(l->n_cols_split - 1);
+    k.reserve(k.size() + 16 + sizeof(int32_t) * ((size_t)l->n_rows_split + (size_t)l->n_cols_split + cells) +
+              sizeof(g2g_block) * (size_t)l->n_blocks);
     put(k, &l->n_rows_split, sizeof l->n_rows_split);
     put(k, &l->n_cols_split, sizeof l->n_cols_split);
     put(k, &l->n_ranks, sizeof l->n_ranks);
     put(k, &l->n_blocks, sizeof l->n_blocks);
     put(k, l->rows_split, sizeof(int32_t) * (size_t)l->n_rows_split);
     put(k, l->cols_split, sizeof(int32_t) * (size_t)l->n_cols_split);
-    put(k, l->owners, sizeof(int32_t) * (size_t)(l->n_rows_split - 1) * (size_t)(l->n_cols_split - 1));
-    for (int32_t b = 0; b < l->n_blocks; ++b) {
-        const g2g_block &x = l->blocks[b];
-        put(k, &x.row, sizeof x.row);
-        put(k, &x.col, sizeof x.col);
-        put(k, &x.stride, sizeof x.stride);
-        const uint64_t d = (uint64_t)(uintptr_t)x.data;
-        put(k, &d, sizeof d);
-    }
+    put(k, l->owners, sizeof(int32_t) * cells);
+    put(k, l->blocks, sizeof(g2g_block) * (size_t)l->n_blocks);
     return true;
 }
 
-// FNV-1a, 64 bit
-uint64_t fnv1a(const std::string &k) {
-    uint64_t h = 1469598103934665603ull;
-    for (unsigned char c : k) {
-        h ^= c;
-        h *= 1099511628211ull;
+// Hash of the key: four independent rotate-multiply lanes over 8-byte words
+// (a large key -- one entry per block -- hashes near memory speed, not at one
+// byte per multiply), mixed with the length at the end. Only a shortcut for the
+// search: a hit also compares the whole key.
+inline uint64_t mix64(uint64_t x) {
+    x *= 0x9e3779b97f4a7c15ull;
+    x ^= x >> 32;
+    x *= 0xd6e8feb86659fd93ull;
+    return x ^ (x >> 29);
+}
+
+uint64_t key_hash(const std::string &k) {
+    const unsigned char *p = (const unsigned char *)k.data();
+    const size_t n = k.size();
+    uint64_t h[4] = {0x243f6a8885a308d3ull, 0x13198a2e03707344ull, 0xa4093822299f31d0ull, 0x082efa98ec4e6c89ull};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32)
+        for (int j = 0; j < 4; ++j) {
+            uint64_t w;
+            std::memcpy(&w, p + i + 8 * j, 8);
+            const uint64_t x = h[j] ^ w;
+            h[j] = (x << 29 | x >> 35) * 0x9e3779b97f4a7c15ull;
+        }
+    for (int j = 0; i < n; i += 8, ++j) {
+        uint64_t w = 0;
+        std::memcpy(&w, p + i, std::min<size_t>(8, n - i));
+        h[j] = mix64(h[j] ^ w);
     }
-    return h;
+    return mix64(h[0] ^ (h[1] << 1 | h[1] >> 63) ^ (h[2] << 2 | h[2] >> 62) ^ (h[3] << 3 | h[3] >> 61) ^ n);
 }
 
 }  // namespace
@@ -109,7 +131,7 @@ int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final_, i
         g2g_plan_destroy(p);
         return internal_fail(G2G_ERR_INVALID, "malformed layout descriptor");
     }
-    const uint64_t h = fnv1a(key);
+    const uint64_t h = key_hash(key);
     plan_cache &c = cache();
     std::shared_ptr<g2g_plan_s> plan;
     {

@@ -526,8 +526,9 @@ hipError_t launch_fill(const fill_block *blocks, int32_t n_blocks, int dtype, ui
 // Streaming reference rates of the device (test support): each thread moves
 // 8 16-byte vectors per grid stride with non-temporal hints; copy reads
 // src and writes dst, read only reads (folding into one value that is
-// written only if it matches a constant), write only writes. No scratch,
-// every access below n vectors.
+// written only if it matches a constant), write only writes. KIND 3 writes
+// with ordinary stores, which allocate in the caches (the tuners' cache
+// eviction). No scratch, every access below n vectors.
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_stream(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
                                                      uint64_t n) {
@@ -539,13 +540,14 @@ __global__ __launch_bounds__(kThreads) void k_stream(const u32x4 *__restrict__ s
         u32x4 r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (KIND != 2 && i + (uint64_t)u * kThreads < n) r[u] = __builtin_nontemporal_load(src + i + u * kThreads);
+            if (KIND < 2 && i + (uint64_t)u * kThreads < n) r[u] = __builtin_nontemporal_load(src + i + u * kThreads);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (i + (uint64_t)u * kThreads >= n) continue;
             if (KIND == 0) __builtin_nontemporal_store(r[u], dst + i + u * kThreads);
             if (KIND == 1) acc ^= r[u];
             if (KIND == 2) __builtin_nontemporal_store(fill, dst + i + u * kThreads);
+            if (KIND == 3) dst[i + u * kThreads] = fill;
         }
     }
     if (KIND == 1 && (acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9e3779b9u && n) dst[0] = acc;
@@ -557,7 +559,8 @@ hipError_t launch_stream(int kind, const void *src, void *dst, uint64_t bytes, i
     u32x4 *d = (u32x4 *)dst;
     if (kind == 0) hipLaunchKernelGGL(k_stream<0>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
     else if (kind == 1) hipLaunchKernelGGL(k_stream<1>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
-    else hipLaunchKernelGGL(k_stream<2>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
+    else if (kind == 2) hipLaunchKernelGGL(k_stream<2>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
+    else hipLaunchKernelGGL(k_stream<3>, dim3((unsigned)grid), dim3(kThreads), 0, stream, s, d, n);
     return hipGetLastError();
 }
 

@@ -70,7 +70,19 @@ class Layout:
         return int(self.cols_split[-1] - self.cols_split[0])
 
     def descriptor(self, base, esize):
-        return Descriptor(self, int(base), esize)
+        """The g2g_layout of this layout at arena address `base`. A layout is
+        not changed after it is built, so the last few descriptors are kept
+        and a repeated call (the drop-in entry point called in a loop) costs
+        no rebuild."""
+        key = (int(base), int(esize))
+        descs = self.__dict__.setdefault("_descs", {})
+        d = descs.get(key)
+        if d is None:
+            d = Descriptor(self, key[0], key[1])
+            if len(descs) >= 8:
+                descs.pop(next(iter(descs)))
+            descs[key] = d
+        return d
 
     def element_mask(self):
         """Boolean mask over the arena: True where an element of a block lives."""

@@ -25,6 +25,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "moved GB/s vs HBM peak (bit-exact); p50/p99 latency, served-batched vs plain"
 
 
+# single-workload legs beside the default workload: buffer pairs rotated, steps, drop-in calls timed
+LEGS = {"bc4096": (8, 400, 80), "irregular": (1, 20, 0)}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +48,8 @@ def parse():
     ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
     ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched leg")
+    ap.add_argument("--only-leg", default=None, choices=sorted(LEGS),
+                    help="run only this single-workload leg (with its buffer rotation; --tuning applies)")
     ap.add_argument("--batch-graph-only", action="store_true",
                     help="batched leg without the per-request comparisons (for counter runs)")
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -135,13 +141,14 @@ def launch_stats(per):
             "min": round(min(per), 4), "max": round(max(per), 4), "launches": len(per)}
 
 
-def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate=1, facade_calls=0):
+def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate=1, facade_calls=0, tuning=None):
     """One workload measured on its own: `rotate` buffer pairs, each with its
     own plan, run round-robin (one step = one transform of one pair), so that
     a working set below the 256 MB Infinity Cache is still read from HBM.
-    The first plan is autotuned; the others start from the tuning it records
-    for their shape. With facade_calls, the drop-in call (transform_layouts)
-    is timed on the same pairs too."""
+    The first plan is autotuned (or set to `tuning`, a tile:threads:unroll:
+    nontemporal:mode[:chunks] string); the others start from the tuning it
+    records for their shape. With facade_calls, the drop-in call
+    (transform_layouts) is timed on the same pairs too."""
     init, fin, dtype, trans = build_workload(name)
     es = g2g.DTYPE_SIZE[dtype]
     pairs = []
@@ -151,9 +158,13 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         tf = torch.zeros(fin.arena_elems * es, dtype=torch.uint8, device=dev)
         g2g.fill_layout(init, ti.data_ptr(), dtype, 100 + k, stream)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans, stream)
-        if k == 0:
-            tuning = plan.autotune(stream=stream, reps=3)
+        if tuning is not None:
+            f = [int(x) for x in tuning.split(":")]
+            plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0)
+        elif k == 0:
+            plan.autotune(stream=stream, reps=3)
         pairs.append((ti, tf, plan))
+    tuning = pairs[0][2].tuning
     later = pairs[-1][2].tuning
     log(f"leg {name}: {rotate} buffer pair(s), planned and tuned in {time.time() - t0:.1f}s: {tuning}")
     it = [0]
@@ -528,6 +539,13 @@ def main():
         print(json.dumps({"batched": out}), flush=True)
         return
 
+    if args.only_leg:
+        rotate, steps, _ = LEGS[args.only_leg]
+        out = workload_leg(torch, g2g, workloads, args.only_leg, stream, dev, max(args.steps, rotate), args.warmup,
+                           rotate=rotate, tuning=args.tuning)
+        print(json.dumps({"leg": args.only_leg, **out}), flush=True)
+        return
+
     init, fin, dtype, trans = build_workload(args.workload)
     es = g2g.DTYPE_SIZE[dtype]
     log(f"workload {args.workload}: {WORKLOAD_DESC.get(args.workload, args.workload)}")
@@ -629,7 +647,7 @@ def main():
         # the other single-transform workloads, each with its own roofline;
         # 4096^2 (2 x 128 MiB) rotates 8 buffer pairs (2 GiB) past the 256 MB Infinity Cache
         result["legs"] = {}
-        for name, rotate, steps, calls in (("bc4096", 8, 400, 80), ("irregular", 1, args.steps, 0)):
+        for name, (rotate, steps, calls) in LEGS.items():
             if name != args.workload:
                 result["legs"][name] = workload_leg(torch, g2g, workloads, name, stream, dev, steps, args.warmup,
                                                     rotate=rotate, facade_calls=calls)

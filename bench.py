@@ -663,6 +663,14 @@ def main():
             log(f"cpu baseline: {result['cpu_baseline']}")
         except Exception as e:  # the baseline is reported, never the target
             result["cpu_baseline"] = {"value": None, "error": str(e)}
+        cb, fa = result["cpu_baseline"], result.get("facade")
+        if cb.get("value") and cb.get("kind") == "reference" and fa and args.workload == "bc32768":
+            # the same basis on both sides: one call that plans, moves and
+            # returns when done (the reference plans on every call)
+            gpu_ms = fa["uncached"]["ms_per_call_median"]
+            cb["same_basis"] = {"gpu_ms_per_call_planning_included": gpu_ms,
+                                "cpu_ms_per_transform": cb["ms_per_transform"],
+                                "gpu_over_cpu": round(cb["ms_per_transform"] / gpu_ms, 1)}
     print(json.dumps(result), flush=True)
 
 

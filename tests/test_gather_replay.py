@@ -122,7 +122,20 @@ def test_replay_mixed_requests():
         _replay(init, fin, dtype, trans, tile_bytes=65536, threads=512)
 
 
-@pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb")])
+@pytest.mark.parametrize("dtype", [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX128])
+def test_replay_irregular_small(dtype):
+    """The irregular benchmark construction at 6000 x 400 (ragged splits,
+    padded leading dimensions, shuffled blocks: columns start at every dword
+    of a 128-byte line), on the pieces and on column chunks."""
+    init, fin = shapes.irregular_small()
+    for tb, th in CONFIGS:
+        st = _replay(init, fin, dtype, tile_bytes=tb, threads=th)
+        assert st[1] > 0, st
+    _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=16)
+
+
+@pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb"),
+                                        (1, "irr"), (2, "irr"), (6, "irr")])
 def test_replay_catches_planted_mistakes(plant, case):
     """A typical indexing mistake planted into the replayed arithmetic (the
     copy queue's last slot not pulled back, leading dimensions swapped, an
@@ -135,6 +148,9 @@ def test_replay_catches_planted_mistakes(plant, case):
     elif case == "cut":
         init, fin = shapes.transposed_pairs()["bcT_cut_edges"]
         args = (init, fin, g2g.FLOAT64, "T", False, 1 << 20, 512, FEAT_TRANS, 0)
+    elif case == "irr":
+        init, fin = shapes.irregular_small()
+        args = (init, fin, g2g.FLOAT32, "N", False, 65536, 512, FEAT_COPY, 0)
     else:
         init, fin = shapes.layout_pairs()["arbitrary"]
         args = (init, fin, g2g.FLOAT64, "N", False, 4096, 512, FEAT_COPY, 0)

@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
                     help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode"
-                         "[:column_chunks] (skips the autotuner)")
+                         "[:column_chunks[:stream_order]] (skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -171,7 +171,8 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans, stream)
         if tuning is not None:
             f = [int(x) for x in tuning.split(":")]
-            plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0)
+            plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
+                            stream_order=f[6] if len(f) > 6 else 0)
         elif k == 0:
             plan.autotune(stream=stream, reps=3)
         pairs.append((ti, tf, plan))
@@ -205,9 +206,10 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         "pieces": int(info.n_pieces),
         "work_items": int(info.n_tiles),
         "tuning": {k: tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                          "column_chunks")},
+                                          "column_chunks", "stream_order")},
         "later_pairs_start_tuned": all(later[k] == tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes",
-                                                                         "nontemporal", "launch_mode")),
+                                                                         "nontemporal", "launch_mode",
+                                                                         "stream_order")),
         "launch_ms": launch_stats(per),
         "roofline": roofline(nbytes, statistics.mean(per), name),
     }
@@ -584,7 +586,8 @@ def main():
     tuning = plan.tuning
     if args.tuning:
         f = [int(x) for x in args.tuning.split(":")]
-        plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0)
+        plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
+                            stream_order=f[6] if len(f) > 6 else 0)
         tuning = plan.tuning
         info = plan.info
     elif args.mode == "fused" and not args.no_autotune:
@@ -626,7 +629,7 @@ def main():
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
             "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                                      "column_chunks")},
+                                                      "column_chunks", "stream_order")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

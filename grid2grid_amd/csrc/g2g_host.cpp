@@ -326,9 +326,10 @@ struct g2g_plan_s {
     std::vector<span> urows, ucols;  // host copy of the interval table
 
     stage_table fused;               // all pieces in union-cell order (wire order, plain form)
-    stage_table chunk;               // column-chunked pieces (gather stream when use_chunk)
-    bool use_chunk = false;
-    uint32_t chunk_w = 0;            // columns per chunk
+    stage_table alt;                 // the gather's stream when it is not `fused` itself (use_alt)
+    bool use_alt = false;
+    uint32_t alt_w = 0;              // alt: columns per chunk (0: whole pieces)
+    uint32_t alt_order = 0;          // alt: G2G_ORDER_* of the stream
     uint64_t stream_gen = 0;         // changes whenever the gather's piece table changes
     uint64_t class_key = 0;          // shape class of the tuning table
     double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
@@ -371,7 +372,7 @@ namespace {
 
 // The table the fused gather runs over: the column-chunked stream when the
 // plan has one, else the pieces themselves.
-stage_table &gtab(g2g_plan p) { return p->use_chunk ? p->chunk : p->fused; }
+stage_table &gtab(g2g_plan p) { return p->use_alt ? p->alt : p->fused; }
 
 // The table a request runs over: scaled requests take the pieces in their own
 // order, whose rows and columns are the whole piece's (the scaled epilogue's
@@ -416,6 +417,7 @@ struct tuned_cfg {
     launch_cfg cfg;
     double ms;
     uint32_t chunk_w = 0;  // > 0: the gather runs over column chunks of this width
+    uint32_t order = 0;    // G2G_ORDER_* of the gather's stream
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
@@ -718,67 +720,89 @@ uint64_t chunk_count(const g2g_plan_s *p, uint32_t w) {
     return g * p->urows.size();
 }
 
-// The gather's stream of a plan: w = 0, the pieces in their own order;
-// w > 0, the pieces cut into chunks of at most w final columns, ordered
-// chunk after chunk down each column interval (built on first use, rebuilt
-// for another width). Vertically adjacent pieces then move the same columns
-// at about the same time: their destination segments, contiguous within a
-// final block column, are written together (longer DRAM runs, and the cache
-// line two pieces share when a segment does not end on a 128-byte line is
-// merged in L2 instead of reaching memory partly written, twice).
-int select_chunks(g2g_plan p, uint32_t w, hipStream_t s) {
-    if (w == 0 || p->fused.n == 0) {
-        if (p->use_chunk) ++p->stream_gen;
-        p->use_chunk = false;
+// The gather's stream of a plan: the pieces (w = 0) or the pieces cut into
+// chunks of at most w final columns, ordered chunk after chunk down each
+// column interval (w > 0), in that order (G2G_ORDER_UNION) or ordered by
+// destination address (G2G_ORDER_DESTINATION). Built on first use, rebuilt
+// for another (w, order). Chunks: vertically adjacent pieces then move the
+// same columns at about the same time: their destination segments, contiguous
+// within a final block column, are written together (longer DRAM runs, and the
+// cache line two pieces share when a segment does not end on a 128-byte line
+// is merged in L2 instead of reaching memory partly written, twice).
+// Destination order: work items that run at the same time write neighbouring
+// destination addresses, final block after final block.
+int select_stream(g2g_plan p, uint32_t w, uint32_t order, hipStream_t s) {
+    if ((w == 0 && order == G2G_ORDER_UNION) || p->fused.n == 0) {
+        if (p->use_alt) ++p->stream_gen;
+        p->use_alt = false;
         return G2G_OK;
     }
-    if (p->chunk.n && p->chunk_w == w) {
-        if (!p->use_chunk) ++p->stream_gen;
-        p->use_chunk = true;
+    if (p->alt.n && p->alt_w == w && p->alt_order == order) {
+        if (!p->use_alt) ++p->stream_gen;
+        p->use_alt = true;
         return G2G_OK;
     }
     ++p->stream_gen;  // batches built on the old table must not run it
-    p->use_chunk = false;
-    p->chunk.pieces.release();
-    p->chunk.pend.release();
-    p->chunk.first.release();
-    p->chunk.n = 0;
-    p->chunk.tile_bytes = 0;
-    p->chunk_w = w;
-    const int64_t n_ur = (int64_t)p->urows.size();
-    std::vector<uint32_t> cuj, cc0;
-    for (size_t uj = 0; uj < p->ucols.size(); ++uj) {
-        const int32_t cols = p->ucols[uj].end - p->ucols[uj].start;
-        for (int32_t c0 = 0; c0 < cols; c0 += (int32_t)w) {
-            cuj.push_back((uint32_t)uj);
-            cc0.push_back((uint32_t)c0);
+    p->use_alt = false;
+    p->alt.pieces.release();
+    p->alt.pend.release();
+    p->alt.first.release();
+    p->alt.n = 0;
+    p->alt.tile_bytes = 0;
+    p->alt_w = w;
+    p->alt_order = order;
+    // the stream's pieces, before ordering: chunks of w columns, or the pieces
+    dbuf<piece> base;
+    dbuf<uint64_t> base_bytes;
+    int64_t n = p->fused.n;
+    if (w) {
+        const int64_t n_ur = (int64_t)p->urows.size();
+        std::vector<uint32_t> cuj, cc0;
+        for (size_t uj = 0; uj < p->ucols.size(); ++uj) {
+            const int32_t cols = p->ucols[uj].end - p->ucols[uj].start;
+            for (int32_t c0 = 0; c0 < cols; c0 += (int32_t)w) {
+                cuj.push_back((uint32_t)uj);
+                cc0.push_back((uint32_t)c0);
+            }
         }
+        const int64_t n_chunks = (int64_t)cuj.size();
+        n = n_chunks * n_ur;
+        // sub-piece indices, plus the 64 examined at once by a wave, stay within 32 bits
+        if (n > 0xffffff00ll)
+            return fail(G2G_ERR_UNSUPPORTED, "%lld column chunks of %u columns: too many", (long long)n, w);
+        dbuf<uint32_t> d_cuj, d_cc0;
+        G2G_TRY(d_cuj.upload(cuj.data(), cuj.size(), s));
+        G2G_TRY(d_cc0.upload(cc0.data(), cc0.size(), s));
+        G2G_TRY(base.alloc((size_t)n));
+        G2G_TRY(base_bytes.alloc((size_t)n));
+        G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
+                                    p->trans != 'N', base.p, base_bytes.p, s));
+        G2G_HIP(hipStreamSynchronize(s));  // the chunk tables go out of scope
     }
-    const int64_t n_chunks = (int64_t)cuj.size(), n = n_chunks * n_ur;
-    // sub-piece indices, plus the 64 examined at once by a wave, stay within 32 bits
-    if (n > 0xffffff00ll) return fail(G2G_ERR_UNSUPPORTED, "%lld column chunks of %u columns: too many",
-                                      (long long)n, w);
-    dbuf<uint32_t> d_cuj, d_cc0;
+    const piece *src = w ? base.p : p->fused.pieces.p;
+    G2G_TRY(p->alt.pieces.alloc((size_t)n));
+    G2G_TRY(p->alt.pend.alloc((size_t)n));
+    if (order == G2G_ORDER_DESTINATION) {
+        G2G_TRY(sort_pieces_by_dst(src, n, p->es, p->alt.pieces.p, p->alt.pend.p, s));
+    } else {  // chunks in their own order
+        G2G_HIP(hipMemcpyAsync(p->alt.pieces.p, base.p, (size_t)n * sizeof(piece), hipMemcpyDeviceToDevice, s));
+        G2G_HIP(hipMemcpyAsync(p->alt.pend.p, base_bytes.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                               s));
+    }
     dbuf<uint64_t> scratch;
-    G2G_TRY(d_cuj.upload(cuj.data(), cuj.size(), s));
-    G2G_TRY(d_cc0.upload(cc0.data(), cc0.size(), s));
-    G2G_TRY(p->chunk.pieces.alloc((size_t)n));
-    G2G_TRY(p->chunk.pend.alloc((size_t)n));
-    G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
-                                p->trans != 'N', p->chunk.pieces.p, p->chunk.pend.p, s));
     G2G_TRY(scratch.alloc(scan_scratch_elems(n)));
-    G2G_HIP(inclusive_scan_u64(p->chunk.pend.p, n, scratch.p, s));
+    G2G_HIP(inclusive_scan_u64(p->alt.pend.p, n, scratch.p, s));
     uint64_t check = 0;
-    if (n) G2G_HIP(hipMemcpyAsync(&check, p->chunk.pend.p + n - 1, sizeof check, hipMemcpyDeviceToHost, s));
+    if (n) G2G_HIP(hipMemcpyAsync(&check, p->alt.pend.p + n - 1, sizeof check, hipMemcpyDeviceToHost, s));
     G2G_HIP(hipStreamSynchronize(s));
     if (check != p->fused.total)
-        return fail(G2G_ERR_INVALID, "internal: chunks cover %llu bytes, matrix has %llu",
+        return fail(G2G_ERR_INVALID, "internal: the stream covers %llu bytes, matrix has %llu",
                     (unsigned long long)check, (unsigned long long)p->fused.total);
-    p->chunk.n = n;  // built
-    p->chunk.total = p->fused.total;
-    p->chunk.tile_bytes = 0;  // not tiled yet
-    p->chunk.cfg = p->fused.cfg;
-    p->use_chunk = true;
+    p->alt.n = n;  // built
+    p->alt.total = p->fused.total;
+    p->alt.tile_bytes = 0;  // not tiled yet
+    p->alt.cfg = p->fused.cfg;
+    p->use_alt = true;
     return G2G_OK;
 }
 
@@ -853,7 +877,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     p->class_key = shape_class(es, trans, total, np);
     tuned_cfg tc;
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
-        if (tc.chunk_w) G2G_TRY(select_chunks(p, tc.chunk_w, s));
+        if (tc.chunk_w || tc.order) G2G_TRY(select_stream(p, tc.chunk_w, tc.order, s));
         stage_table &gt = gtab(p);
         G2G_TRY(retile(gt, tc.tile_bytes, s));
         gt.cfg = tc.cfg;
@@ -864,7 +888,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     }
     // scaled requests run over the pieces themselves: tiled from the start,
     // so that no launch has to build anything
-    if (p->use_chunk) {
+    if (p->use_alt) {
         G2G_TRY(retile(p->fused, choose_tile_bytes(total), s));
         p->fused.cfg = default_cfg(p->fused.n_tiles);
     }
@@ -886,8 +910,8 @@ int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
     t->unroll_bytes = gt.cfg.unroll_bytes;
     t->nontemporal = gt.cfg.nontemporal;
     t->launch_mode = launch_mode_of(gt.cfg);
-    t->column_chunks = p->use_chunk ? (int32_t)p->chunk_w : 0;
-    t->reserved = 0;
+    t->column_chunks = p->use_alt ? (int32_t)p->alt_w : 0;
+    t->stream_order = p->use_alt ? (int32_t)p->alt_order : G2G_ORDER_UNION;
     t->ms = p->tuned_ms;
     return G2G_OK;
 }
@@ -899,8 +923,10 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
         t->launch_mode > G2G_LAUNCH_PERSISTENT_XCD)
         return fail(G2G_ERR_INVALID, "tuning outside the compiled variants");
     if (t->column_chunks < 0) return fail(G2G_ERR_INVALID, "column_chunks %d < 0", (int)t->column_chunks);
+    if (t->stream_order != G2G_ORDER_UNION && t->stream_order != G2G_ORDER_DESTINATION)
+        return fail(G2G_ERR_INVALID, "unknown stream_order %d", (int)t->stream_order);
     std::lock_guard<std::mutex> lock(p->mu);
-    G2G_TRY(select_chunks(p, (uint32_t)t->column_chunks, nullptr));
+    G2G_TRY(select_stream(p, (uint32_t)t->column_chunks, (uint32_t)t->stream_order, nullptr));
     stage_table &gt = gtab(p);
     if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
     gt.cfg.threads = t->threads;
@@ -996,18 +1022,19 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     std::memset(&kr, 0, sizeof kr);
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
-    // the sweep runs over the pieces in their own order; the fastest few
-    // configurations are then timed again on it and on column-chunked streams
-    // of several widths (transposed plans: whole groups of 256 bytes of final
-    // columns, the width of a transpose tile of 64 final rows)
+    // The sweep runs over the pieces, in their own order and in destination
+    // order; the fastest few configurations of each order are then timed
+    // again on it and on column-chunked streams of several widths in the same
+    // order (transposed plans: whole groups of 256 bytes of final columns,
+    // the width of a transpose tile of 64 final rows).
     std::lock_guard<std::mutex> lock(p->mu);
-    if (p->use_chunk) ++p->stream_gen;  // batches built on the chunk stream must not run it
-    p->use_chunk = false;
+    if (p->use_alt) ++p->stream_gen;  // batches built on another stream must not run it
+    p->use_alt = false;
     const uint64_t total = p->fused.total;
     std::vector<uint64_t> sizes;
-    for (uint64_t tb = 32768; tb <= (1u << 20); tb *= 2)
+    for (uint64_t tb : {32768u, 40960u, 49152u, 65536u, 98304u, 131072u, 262144u, 524288u, 1048576u})
         if (sizes.empty() || tb < total * 2) sizes.push_back(tb);
-    // time one configuration (the plan is retiled to tb when it differs)
+    // time one configuration on the current stream (retiled to tb when it differs)
     auto time_cfg = [&](uint64_t tb, const launch_cfg &c, int n, double &ms_out) -> int {
         stage_table &gt = gtab(p);
         if (gt.tile_bytes != tb) G2G_TRY(retile(gt, tb, s));
@@ -1024,53 +1051,57 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
             },
             n, ms_out);
     };
-    // sweep every variant, then time the few fastest again with more runs,
-    // so that one noisy measurement does not pick the winner
-    std::vector<tuned_cfg> cand;
-    int rc = G2G_OK;
-    for (uint64_t tb : sizes) {
-        for (int th : {256, 512})
-            for (int ub : {64, 128})
-                for (int nt = 0; nt < 2; ++nt)
-                    for (int mode : {G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_DIRECT_XCD,
-                                     G2G_LAUNCH_PERSISTENT_XCD}) {
-                        launch_cfg c;
-                        c.threads = th;
-                        c.unroll_bytes = ub;
-                        c.nontemporal = nt != 0;
-                        set_launch_mode(c, mode);
-                        c.features = request_features(kr.r[0]);
-                        double ms = 0;
-                        if ((rc = time_cfg(tb, c, reps, ms)) != G2G_OK) break;
-                        cand.push_back(tuned_cfg{tb, c, ms});
-                    }
-        if (rc != G2G_OK) break;
-    }
+    std::vector<uint32_t> orders{G2G_ORDER_UNION};
+    if (p->fused.n > 1) orders.push_back(G2G_ORDER_DESTINATION);
+    std::vector<uint32_t> widths{0};
+    const uint32_t q = 256 / p->es;
+    for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{8, 16, 32, 64} : std::vector<uint32_t>{q, 2 * q, 4 * q}))
+        if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
     tuned_cfg win{p->fused.tile_bytes, p->fused.cfg, 0};
-    if (rc == G2G_OK && !cand.empty()) {
+    double best_ms = 1e30;
+    int rc = G2G_OK;
+    for (uint32_t order : orders) {
+        // sweep every variant, then time the few fastest again with more
+        // runs, so that one noisy measurement does not pick the winner
+        if ((rc = select_stream(p, 0, order, s)) != G2G_OK) break;
+        std::vector<tuned_cfg> cand;
+        for (uint64_t tb : sizes) {
+            for (int th : {256, 512})
+                for (int ub : {64, 128})
+                    for (int nt = 0; nt < 2; ++nt)
+                        for (int mode : {G2G_LAUNCH_DIRECT, G2G_LAUNCH_PERSISTENT, G2G_LAUNCH_DIRECT_XCD,
+                                         G2G_LAUNCH_PERSISTENT_XCD}) {
+                            launch_cfg c;
+                            c.threads = th;
+                            c.unroll_bytes = ub;
+                            c.nontemporal = nt != 0;
+                            set_launch_mode(c, mode);
+                            c.features = request_features(kr.r[0]);
+                            double ms = 0;
+                            if ((rc = time_cfg(tb, c, reps, ms)) != G2G_OK) break;
+                            cand.push_back(tuned_cfg{tb, c, ms});
+                        }
+            if (rc != G2G_OK) break;
+        }
+        if (rc != G2G_OK || cand.empty()) break;
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
         const size_t top = std::min<size_t>(4, cand.size());
-        double best_ms = 1e30;
-        std::vector<uint32_t> widths{0};
-        const uint32_t q = 256 / p->es;
-        for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{8, 16, 32, 64}
-                                           : std::vector<uint32_t>{q, 2 * q, 4 * q}))
-            if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
         for (uint32_t w : widths) {
-            if ((rc = select_chunks(p, w, s)) != G2G_OK) break;
+            if ((rc = select_stream(p, w, order, s)) != G2G_OK) break;
             for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
                 double ms = 0;
                 if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, w};
+                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, w, order};
                 }
             }
             if (rc != G2G_OK) break;
         }
+        if (rc != G2G_OK) break;
     }
     if (rc != G2G_OK) return rc;
-    G2G_TRY(select_chunks(p, win.chunk_w, s));
+    G2G_TRY(select_stream(p, win.chunk_w, win.order, s));
     stage_table &gt = gtab(p);
     G2G_TRY(retile(gt, win.tile_bytes, s));
     gt.cfg = win.cfg;
@@ -1507,7 +1538,7 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->nontemporal = b->cfg.nontemporal;
         best->launch_mode = launch_mode_of(b->cfg);
         best->column_chunks = 0;
-        best->reserved = 0;
+        best->stream_order = G2G_ORDER_UNION;
         best->ms = b->tuned_ms;
     }
     return G2G_OK;
@@ -1521,7 +1552,7 @@ int g2g_batch_get_tuning(g2g_batch b, g2g_tuning *t) {
     t->nontemporal = b->cfg.nontemporal;
     t->launch_mode = launch_mode_of(b->cfg);
     t->column_chunks = 0;
-    t->reserved = 0;
+    t->stream_order = G2G_ORDER_UNION;
     t->ms = b->tuned_ms;
     return G2G_OK;
 }
@@ -1532,7 +1563,8 @@ int g2g_batch_set_tuning(g2g_batch b, const g2g_tuning *t, void *stream) {
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) ||
         (t->launch_mode != G2G_LAUNCH_PERSISTENT && t->launch_mode != G2G_LAUNCH_PERSISTENT_XCD))
         return fail(G2G_ERR_INVALID, "batch tuning outside the compiled variants (batches launch persistent)");
-    if (t->column_chunks != 0) return fail(G2G_ERR_INVALID, "a batch runs each plan's own stream: column_chunks must be 0");
+    if (t->column_chunks != 0 || t->stream_order != G2G_ORDER_UNION)
+        return fail(G2G_ERR_INVALID, "a batch runs each plan's own stream: column_chunks and stream_order must be 0");
     if ((uint64_t)t->tile_bytes != b->tile_bytes) G2G_TRY(batch_retile(b, (uint64_t)t->tile_bytes, (hipStream_t)stream));
     b->drop_graph();
     b->cfg.threads = t->threads;

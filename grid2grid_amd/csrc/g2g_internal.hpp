@@ -119,6 +119,10 @@ hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
                                const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t esize,
                                int trans, piece *out, uint64_t *bytes, hipStream_t stream);
 
+// out = the n pieces of `in` ordered by destination address, bytes[k] =
+// out[k].rows * cols * es. Returns a g2g_status; synchronous.
+int sort_pieces_by_dst(const piece *in, int64_t n, uint32_t es, piece *out, uint64_t *bytes, hipStream_t stream);
+
 // bytes[p] = rows*cols*esize of the piece's request.
 hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const kreqs &kr, uint64_t *bytes,
                               hipStream_t stream);

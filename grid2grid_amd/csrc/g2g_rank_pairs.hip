@@ -231,6 +231,51 @@ int launch_pair_offsets_sorted(const uint32_t *ranks, const piece *pieces, int64
     return G2G_OK;
 }
 
+namespace {
+
+__global__ __launch_bounds__(256) void k_dst_keys(const piece *in, int64_t n, uint64_t *keys, uint32_t *idx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    keys[k] = in[k].dst;
+    idx[k] = (uint32_t)k;
+}
+
+__global__ __launch_bounds__(256) void k_gather_pieces(const piece *in, const uint32_t *sidx, int64_t n,
+                                                       uint32_t es, piece *out, uint64_t *bytes) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const piece pc = in[sidx[k]];
+    out[k] = pc;
+    bytes[k] = (uint64_t)pc.rows * pc.cols * es;
+}
+
+}  // namespace
+
+// out = the n pieces of `in` ordered by destination address (a stable radix
+// sort of the 64-bit addresses), bytes[k] = out[k]'s bytes. Synchronous on
+// `s` (scratch is freed on return).
+int sort_pieces_by_dst(const piece *in, int64_t n, uint32_t es, piece *out, uint64_t *bytes, hipStream_t s) {
+    if (n == 0) return G2G_OK;
+    if (n > 0x7fffffffll) return internal_fail(G2G_ERR_UNSUPPORTED, "destination order: %lld pieces", (long long)n);
+    scratch_set sc;
+    uint64_t *keys = sc.get<uint64_t>(n), *skeys = sc.get<uint64_t>(n);
+    uint32_t *idx = sc.get<uint32_t>(n), *sidx = sc.get<uint32_t>(n);
+    if (!keys || !skeys || !idx || !sidx) return internal_fail(G2G_ERR_NOMEM, "destination order: device allocation failed");
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    const int ni = (int)n;
+    hipLaunchKernelGGL(k_dst_keys, dim3(gn), dim3(256), 0, s, in, n, keys, idx);
+    VOL_HIP(hipGetLastError());
+    size_t b = 0;
+    VOL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, keys, skeys, idx, sidx, ni, 0, 64, s));
+    void *tmp = sc.get<unsigned char>(b);
+    if (!tmp) return internal_fail(G2G_ERR_NOMEM, "destination order: device allocation failed");
+    VOL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, skeys, idx, sidx, ni, 0, 64, s));
+    hipLaunchKernelGGL(k_gather_pieces, dim3(gn), dim3(256), 0, s, in, sidx, n, es, out, bytes);
+    VOL_HIP(hipGetLastError());
+    VOL_HIP(hipStreamSynchronize(s));
+    return G2G_OK;
+}
+
 }  // namespace g2g
 
 extern "C" {

@@ -35,8 +35,10 @@ extern "C" {
 /* 1: plans, batches, staged form, volume, reordering, serving loop.
  * 2 adds (nothing removed or changed): g2g_transform_layouts and the plan
  * cache, g2g_transform_graph, g2g_scale_layout, g2g_copy_bytes,
- * g2g_batch_get/set_tuning, g2g_transform_plans_noalloc. */
-#define G2G_ABI_VERSION 2
+ * g2g_batch_get/set_tuning, g2g_transform_plans_noalloc.
+ * 3 adds g2g_tuning.stream_order (the word version 2 reserved as 0) and
+ * G2G_ORDER_*. */
+#define G2G_ABI_VERSION 3
 
 /* Element types; the reference instantiates transform for these four
  * (src/grid2grid/transform.cpp:206-298). */
@@ -340,6 +342,9 @@ enum {
     G2G_LAUNCH_PERSISTENT_XCD = 3
 };
 
+/* Order of a plan's stream (g2g_tuning.stream_order). */
+enum { G2G_ORDER_UNION = 0, G2G_ORDER_DESTINATION = 1 };
+
 /* Launch configuration of a plan's fused gather. */
 typedef struct g2g_tuning {
     int64_t tile_bytes;    /* bytes of the piece stream per work item */
@@ -355,7 +360,14 @@ typedef struct g2g_tuning {
      * Batches run each plan's stream as the plan is tuned (their own tuning
      * reports 0). */
     int32_t column_chunks;
-    int32_t reserved;      /* 0 */
+    /* G2G_ORDER_UNION: the stream in union-cell order (pieces, or chunks of
+     * the pieces as above); G2G_ORDER_DESTINATION: the same stream ordered
+     * by destination address, so that work items that run at the same time
+     * write neighbouring addresses, final block after final block. Batches
+     * report and accept only G2G_ORDER_UNION for their own tuning (each
+     * plan's stream runs as the plan is tuned). This word was reserved (0)
+     * before ABI version 3: such callers get the union order. */
+    int32_t stream_order;
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
 } g2g_tuning;
 
@@ -366,6 +378,7 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
  * non-temporal hint, launch mode) at several work-item sizes on this plan
  * (`reps` timed runs each, on `stream`, hipEvents), time the fastest few
  * again, on the pieces and on column-chunked streams of several widths,
+ * each in union-cell and in destination order,
  * keep the fastest for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform

@@ -283,46 +283,52 @@ def test_transposed_whole_tiles(dtype):
 @pytest.mark.parametrize("name", ["bc_small", "bc_grid_change", "arbitrary", "tall_thin", "many_ranks"])
 def test_column_chunked_stream(name):
     """The gather over column chunks (pieces cut into chunks of w columns,
-    ordered down each column interval) writes exactly what the pieces in
-    their own order write, for odd widths and every launch mode; the width is
-    reported back; a negative width is refused."""
+    ordered down each column interval) and over the pieces or chunks in
+    destination order writes exactly what the pieces in their own order
+    write, for odd widths, work-item sizes that are not powers of two and
+    every launch mode; the width and order are reported back; a negative
+    width and an unknown order are refused."""
     init, fin = _layout_pairs()[name]
     dtype = g2g.FLOAT32
     rng = np.random.default_rng(17)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
     want = _expected(init, ia, fin, fa, dtype)
     ti = _dev(ia)
-    for w, mode, tile in ((1, 0, 4096), (3, 1, 65536), (16, 2, 4096), (64, 3, 1 << 20), (3, 0, 1 << 20)):
-        print(f"chunks {name} w {w} mode {mode} tile {tile}", flush=True)
+    for w, mode, tile, order in ((1, 0, 4096, 0), (3, 1, 65536, 1), (16, 2, 4096, 0), (64, 3, 1 << 20, 1),
+                                 (3, 0, 1 << 20, 0), (0, 1, 40960, 1), (0, 0, 4096, 1), (0, 3, 49152, 1),
+                                 (16, 1, 12288, 1)):
+        print(f"chunks {name} w {w} mode {mode} tile {tile} order {order}", flush=True)
         tf = _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
-        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w)
-        assert plan.tuning["column_chunks"] == w
+        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w, stream_order=order)
+        assert plan.tuning["column_chunks"] == w and plan.tuning["stream_order"] == order
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         # back to the pieces in their own order
         tf.copy_(_dev(fa))
         plan.set_tuning(tile, 256, 64, 1, mode, column_chunks=0)
-        assert plan.tuning["column_chunks"] == 0
+        assert plan.tuning["column_chunks"] == 0 and plan.tuning["stream_order"] == 0
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         plan.close()
-    with pytest.raises(g2g.G2GError):
-        plan = g2g.Plan(init, ti.data_ptr(), fin, ti.data_ptr(), dtype)
-        try:
-            plan.set_tuning(65536, 256, 64, 0, 0, column_chunks=-1)
-        finally:
-            plan.close()
+    for bad in ({"column_chunks": -1}, {"stream_order": 2}, {"stream_order": -1}):
+        with pytest.raises(g2g.G2GError):
+            plan = g2g.Plan(init, ti.data_ptr(), fin, ti.data_ptr(), dtype)
+            try:
+                plan.set_tuning(65536, 256, 64, 0, 0, **bad)
+            finally:
+                plan.close()
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("name", ["bcT", "arbT", "bcT_mixed_paths", "bcT_short_tiles", "bcT_mid_tiles", "bcT_cut_edges"])
 def test_column_chunked_transposed(name, dtype):
     """Transposed pieces cut into chunks of final columns (source offsets
-    along the storage rows): whole LDS tiles, strips and the element queue
-    give the oracle's bytes for chunk widths that split tiles and groups."""
+    along the storage rows), in union and in destination order: whole LDS
+    tiles, strips and the element queue give the oracle's bytes for chunk
+    widths that split tiles and groups."""
     init, fin = _transposed_pairs()[name]
     rng = np.random.default_rng(41)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
@@ -330,12 +336,13 @@ def test_column_chunked_transposed(name, dtype):
     want = _expected(init, ia, fin, fa, dtype, trans)
     ti = _dev(ia)
     q = 256 // g2g.DTYPE_SIZE[dtype]
-    for w, mode, tile in ((q, 0, 65536), (3, 1, 4096), (2 * q, 2, 1 << 20), (40, 3, 65536)):
-        print(f"transposed chunks {name} dtype {dtype} w {w} mode {mode}", flush=True)
+    for w, mode, tile, order in ((q, 0, 65536, 0), (3, 1, 4096, 1), (2 * q, 2, 1 << 20, 0), (40, 3, 65536, 0),
+                                 (0, 1, 49152, 1), (q, 3, 40960, 1)):
+        print(f"transposed chunks {name} dtype {dtype} w {w} mode {mode} order {order}", flush=True)
         tf = _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
-        plan.set_tuning(tile, 512, 128, 1, mode, column_chunks=w)
-        assert plan.tuning["column_chunks"] == w
+        plan.set_tuning(tile, 512, 128, 1, mode, column_chunks=w, stream_order=order)
+        assert plan.tuning["column_chunks"] == w and plan.tuning["stream_order"] == order
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
@@ -343,8 +350,9 @@ def test_column_chunked_transposed(name, dtype):
 
 
 def test_column_chunked_batch_and_autotune():
-    """A batch of chunked and unchunked plans gives every request's own
-    result; the autotuner's choice (chunked or not) too."""
+    """A batch of chunked and unchunked plans, in union and destination
+    order, gives every request's own result; the autotuner's choice (chunked
+    or not, either order) too."""
     dtype = g2g.FLOAT64
     rng = np.random.default_rng(23)
     cases = []
@@ -354,7 +362,9 @@ def test_column_chunked_batch_and_autotune():
         ti, tf = _dev(ia), _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
         if k % 2 == 0:
-            plan.set_tuning(65536, 256, 64, 0, 1, column_chunks=2 + k)
+            plan.set_tuning(65536, 256, 64, 0, 1, column_chunks=2 + k, stream_order=k // 2)
+        elif k == 3:
+            plan.set_tuning(40960, 512, 128, 0, 1, stream_order=1)
         cases.append((plan, ti, tf, fa, _expected(init, ia, fin, fa, dtype)))
     batch = g2g.Batch([c[0] for c in cases])
     batch.execute()

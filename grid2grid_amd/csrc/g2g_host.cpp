@@ -1023,7 +1023,7 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
     // The sweep runs over the pieces, in their own order and in destination
-    // order; the fastest few configurations of each order are then timed
+    // order; the eight fastest configurations of each order are then timed
     // again on it and on column-chunked streams of several widths in the same
     // order (transposed plans: whole groups of 256 bytes of final columns,
     // the width of a transpose tile of 64 final rows).
@@ -1055,7 +1055,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     if (p->fused.n > 1) orders.push_back(G2G_ORDER_DESTINATION);
     std::vector<uint32_t> widths{0};
     const uint32_t q = 256 / p->es;
-    for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{8, 16, 32, 64} : std::vector<uint32_t>{q, 2 * q, 4 * q}))
+    for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{1, 2, 4, 8, 16, 32, 64}
+                                       : std::vector<uint32_t>{q, 2 * q, 4 * q}))
         if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
     tuned_cfg win{p->fused.tile_bytes, p->fused.cfg, 0};
     double best_ms = 1e30;
@@ -1085,7 +1086,7 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         }
         if (rc != G2G_OK || cand.empty()) break;
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
-        const size_t top = std::min<size_t>(4, cand.size());
+        const size_t top = std::min<size_t>(8, cand.size());
         for (uint32_t w : widths) {
             if ((rc = select_stream(p, w, order, s)) != G2G_OK) break;
             for (size_t k = 0; k < top && rc == G2G_OK; ++k) {

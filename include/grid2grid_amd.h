@@ -375,10 +375,10 @@ int g2g_plan_get_tuning(g2g_plan plan, g2g_tuning *t);
 int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
 
 /* Autotuner: time every compiled variant (workgroup size, bytes in flight,
- * non-temporal hint, launch mode) at several work-item sizes on this plan
- * (`reps` timed runs each, on `stream`, hipEvents), time the fastest few
- * again, on the pieces and on column-chunked streams of several widths,
- * each in union-cell and in destination order,
+ * non-temporal hint, launch mode) at several work-item sizes on this plan's
+ * pieces, in union-cell and in destination order (`reps` timed runs each,
+ * on `stream`, hipEvents), time the eight fastest of each order again on the
+ * pieces and on column-chunked streams of several widths in the same order,
  * keep the fastest for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform

@@ -218,7 +218,8 @@ typedef struct g2g_batch_s *g2g_batch;
  * k's dtype, or the arrays are NULL (plain copies). Plans must outlive the
  * batch. The batch runs each plan's piece stream as it was at creation:
  * once a plan's stream changes (column chunks switched on or off or to
- * another width, by g2g_plan_set_tuning or g2g_plan_autotune) the batch's
+ * another width, or the stream order changed, by g2g_plan_set_tuning or
+ * g2g_plan_autotune) the batch's
  * launches fail with G2G_ERR_INVALID until it is built again; a new work-item
  * size or kernel variant of the plan leaves the batch as it is. */
 int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,

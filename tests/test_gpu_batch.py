@@ -207,8 +207,9 @@ def test_server_plain_mode_and_budget_reject():
 
 def test_batch_refuses_after_a_plan_is_retuned():
     """A batch holds its plans' piece tables by address: once a plan is
-    retuned to another stream (column chunks), launching the old batch fails
-    cleanly, and a batch built again gives every request's own result."""
+    retuned to another stream (column chunks, stream order), launching the
+    old batch fails cleanly, and a batch built again gives every request's
+    own result."""
     cases = _cases(6, 9)
     untransposed = [c for c in cases if c.trans == "N"]
     assert untransposed
@@ -231,6 +232,20 @@ def test_batch_refuses_after_a_plan_is_retuned():
     torch.cuda.synchronize()
     for k, c in enumerate(cases):
         c.check(f"rebuilt batch request {k}")
+    # the stream order is part of the stream too
+    t = cases[-1].plan.tuning
+    cases[-1].plan.set_tuning(t["tile_bytes"], t["threads"], t["unroll_bytes"], t["nontemporal"], t["launch_mode"],
+                              column_chunks=t["column_chunks"], stream_order=1 - t["stream_order"])
+    with pytest.raises(g2g.G2GError):
+        batch.execute()
+    batch.close()
+    for c in cases:
+        c.reset()
+    batch = g2g.Batch([c.plan for c in cases])
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, c in enumerate(cases):
+        c.check(f"batch rebuilt after the order change, request {k}")
     batch.close()
 
 

@@ -190,6 +190,7 @@ extern "C" {
 // Replays the fused gather of (initial -> final, dtype, trans) with work items
 // of tile_bytes, workgroups of `threads`, kernel feature level `feat`
 // (FEAT_COPY 0, FEAT_TRANS 1, FEAT_ALL 2), column chunks of chunk_w columns
+// in stream order `order` (G2G_ORDER_UNION or G2G_ORDER_DESTINATION)
 // (0: the pieces), scaled = 1 for a scaled request (the scaled path moves
 // every piece). Returns the number of errors; the first ones are described in
 // msg. stats (may be NULL): work items, copy slots, element slots, tile
@@ -202,7 +203,8 @@ extern "C" {
 // axis, 4 a transpose tile's phase 2 one final column off, 5 the LDS row
 // padding dropped in phase 2, 6 work items reaching one column into the next.
 int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype, char trans, int scaled,
-                  int64_t tile_bytes, int threads, int feat, uint32_t chunk_w, int plant, char *msg, int msg_len,
+                  int64_t tile_bytes, int threads, int feat, uint32_t chunk_w, int order, int plant, char *msg,
+                  int msg_len,
                   int64_t *stats) {
     replay R;
     const uint32_t es = dtype == G2G_FLOAT32 ? 4 : dtype == G2G_COMPLEX128 ? 16 : 8;
@@ -239,6 +241,10 @@ int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype
                     ch.push_back(chunk_piece(pieces[uj * n_ur + ui], (uint32_t)c0, chunk_w, es, R.trans));
         pieces.swap(ch);
     }
+    // destination order (select_stream, sort_pieces_by_dst): a stable sort by
+    // destination address
+    if (order == G2G_ORDER_DESTINATION)
+        std::stable_sort(pieces.begin(), pieces.end(), [](const piece &a, const piece &b) { return a.dst < b.dst; });
     const int64_t np = (int64_t)pieces.size();
     std::vector<uint64_t> pend(np);
     uint64_t total = 0;

@@ -26,13 +26,13 @@ DTYPES = [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64, g2g.COMPLEX128]
 _lib = None
 
 
-def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant):
+def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant, order=0):
     global _lib
     if _lib is None:
         _lib = ctypes.CDLL(LIB)
         _lib.replay_gather.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_char, ctypes.c_int,
                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
-                                       ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+                                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     if feat is None:
         feat = FEAT_ALL if scaled else (FEAT_TRANS if trans != "N" else FEAT_COPY)
     es = g2g.DTYPE_SIZE[dtype]
@@ -42,14 +42,15 @@ def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, pl
     msg = ctypes.create_string_buffer(4096)
     stats = (ctypes.c_int64 * 6)()
     errors = _lib.replay_gather(ctypes.addressof(di.struct), ctypes.addressof(df.struct), dtype, trans.encode(),
-                                int(scaled), tile_bytes, threads, feat, chunk_w, plant, msg, len(msg), stats)
+                                int(scaled), tile_bytes, threads, feat, chunk_w, order, plant, msg, len(msg), stats)
     return errors, msg.value.decode(), list(stats)
 
 
-def _replay(init, fin, dtype, trans="N", scaled=False, tile_bytes=65536, threads=512, feat=None, chunk_w=0):
-    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0)
+def _replay(init, fin, dtype, trans="N", scaled=False, tile_bytes=65536, threads=512, feat=None, chunk_w=0,
+            order=0):
+    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0, order)
     assert errors == 0, (f"{errors} errors ({init.m}x{init.n} dtype {dtype} trans {trans} scaled {scaled} "
-                         f"tile {tile_bytes} threads {threads} feat {feat} chunks {chunk_w}):\n" + msg)
+                         f"tile {tile_bytes} threads {threads} feat {feat} chunks {chunk_w} order {order}):\n" + msg)
     return stats
 
 
@@ -66,6 +67,8 @@ def test_replay_untransposed(name):
         _replay(init, fin, dtype, scaled=True, tile_bytes=4096, threads=256)
         for w in (1, 3, 16):
             _replay(init, fin, dtype, tile_bytes=8192, threads=512, chunk_w=w)
+        for w, tb in ((0, 40960), (2, 12288), (16, 49152)):
+            _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
 
 
 @pytest.mark.parametrize("name", list(shapes.transposed_pairs()))
@@ -80,6 +83,8 @@ def test_replay_transposed(name):
             q = 256 // g2g.DTYPE_SIZE[dtype]
             for w in (3, q, 2 * q):
                 _replay(init, fin, dtype, trans, tile_bytes=65536, threads=512, chunk_w=w)
+            for w, tb in ((0, 49152), (q, 40960)):
+                _replay(init, fin, dtype, trans, tile_bytes=tb, threads=512, chunk_w=w, order=1)
 
 
 def test_replay_cut_edges_take_both_paths():
@@ -110,6 +115,7 @@ def test_replay_benchmark_block_cyclic_4096():
     st = _replay(init, fin, dtype, tile_bytes=65536, threads=512)
     assert st[0] == 2048
     _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=16)
+    _replay(init, fin, dtype, tile_bytes=49152, threads=512, chunk_w=2, order=1)
 
 
 def test_replay_mixed_requests():

@@ -138,6 +138,8 @@ def test_replay_irregular_small(dtype):
         st = _replay(init, fin, dtype, tile_bytes=tb, threads=th)
         assert st[1] > 0, st
     _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=16)
+    for w, tb in ((0, 40960), (2, 49152)):
+        _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
 
 
 @pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb"),

@@ -132,7 +132,8 @@ def test_replay_mixed_requests():
 def test_replay_irregular_small(dtype):
     """The irregular benchmark construction at 6000 x 400 (ragged splits,
     padded leading dimensions, shuffled blocks: columns start at every dword
-    of a 128-byte line), on the pieces and on column chunks."""
+    of a 128-byte line), on the pieces and on column chunks, in union and in
+    destination order."""
     init, fin = shapes.irregular_small()
     for tb, th in CONFIGS:
         st = _replay(init, fin, dtype, tile_bytes=tb, threads=th)

@@ -370,8 +370,9 @@ struct g2g_plan_s {
 
 namespace {
 
-// The table the fused gather runs over: the column-chunked stream when the
-// plan has one, else the pieces themselves.
+// The table the fused gather runs over: the plan's selected stream (column
+// chunks and/or destination order) when it has one, else the pieces
+// themselves in union-cell order.
 stage_table &gtab(g2g_plan p) { return p->use_alt ? p->alt : p->fused; }
 
 // The table a request runs over: scaled requests take the pieces in their own

@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
                     help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode"
-                         "[:column_chunks[:stream_order]] (skips the autotuner)")
+                         "[:column_chunks[:stream_order[:row_chunks]]] (skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -172,7 +172,7 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         if tuning is not None:
             f = [int(x) for x in tuning.split(":")]
             plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
-                            stream_order=f[6] if len(f) > 6 else 0)
+                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0)
         elif k == 0:
             plan.autotune(stream=stream, reps=3)
         pairs.append((ti, tf, plan))
@@ -206,7 +206,7 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         "pieces": int(info.n_pieces),
         "work_items": int(info.n_tiles),
         "tuning": {k: tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                          "column_chunks", "stream_order")},
+                                          "column_chunks", "stream_order", "row_chunks")},
         "later_pairs_start_tuned": all(later[k] == tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes",
                                                                          "nontemporal", "launch_mode",
                                                                          "stream_order")),
@@ -587,7 +587,7 @@ def main():
     if args.tuning:
         f = [int(x) for x in args.tuning.split(":")]
         plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
-                            stream_order=f[6] if len(f) > 6 else 0)
+                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0)
         tuning = plan.tuning
         info = plan.info
     elif args.mode == "fused" and not args.no_autotune:
@@ -629,7 +629,7 @@ def main():
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
             "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                                      "column_chunks", "stream_order")},
+                                                      "column_chunks", "stream_order", "row_chunks")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

@@ -28,7 +28,7 @@ def _scalar_bytes(dtype, value):
 
 
 def _tuning_dict(t):
-    return {k: getattr(t, k) for k, _ in t._fields_}
+    return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
 
 
 def _stream(s):
@@ -121,14 +121,16 @@ class Plan:
         return _tuning_dict(t)
 
     def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, launch_mode=0,
-                   column_chunks=0, stream_order=0):
+                   column_chunks=0, stream_order=0, row_chunks=0):
         """launch_mode: 0 one workgroup per work item, 1 persistent grid, 2 one
         workgroup per work item with a contiguous range of work items per XCD,
         3 persistent with one counter per XCD. column_chunks: 0 runs the
         pieces, w > 0 chunks of at most w columns ordered down each column
-        interval. stream_order: 0 union-cell order, 1 destination order."""
+        interval. stream_order: 0 union-cell order, 1 destination order.
+        row_chunks: 0 whole columns, h > 0 (untransposed plans) chunks of at
+        most h rows."""
         t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode),
-                           int(column_chunks), int(stream_order), 0.0)
+                           int(column_chunks), int(stream_order), int(row_chunks), 0, 0.0)
         check(_lib.load().g2g_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, stream=None, reps=3):
@@ -190,7 +192,8 @@ class Batch:
 
     def set_tuning(self, tile_bytes, threads=512, unroll_bytes=128, nontemporal=0, launch_mode=1, stream=None):
         """launch_mode 1 (one work counter) or 3 (one per XCD); batches always launch persistent."""
-        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode), 0, 0, 0.0)
+        t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode), 0, 0, 0, 0,
+                           0.0)
         check(_lib.load().g2g_batch_set_tuning(self.handle, ctypes.byref(t), _stream(stream)))
 
     def close(self):

@@ -89,6 +89,10 @@ struct dbuf {
         if (count) G2G_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
         return G2G_OK;
     }
+    void swap(dbuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
 };
 
 // ------------------------------------------------------ work counters
@@ -330,6 +334,7 @@ struct g2g_plan_s {
     bool use_alt = false;
     uint32_t alt_w = 0;              // alt: columns per chunk (0: whole pieces)
     uint32_t alt_order = 0;          // alt: G2G_ORDER_* of the stream
+    uint32_t alt_h = 0;              // alt: rows per chunk (0: whole columns)
     uint64_t stream_gen = 0;         // changes whenever the gather's piece table changes
     uint64_t class_key = 0;          // shape class of the tuning table
     double tuned_ms = 0;             // time of the tuned configuration (0 = not tuned)
@@ -419,6 +424,7 @@ struct tuned_cfg {
     double ms;
     uint32_t chunk_w = 0;  // > 0: the gather runs over column chunks of this width
     uint32_t order = 0;    // G2G_ORDER_* of the gather's stream
+    uint32_t chunk_h = 0;  // > 0: ... and row chunks of this height
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
@@ -723,22 +729,28 @@ uint64_t chunk_count(const g2g_plan_s *p, uint32_t w) {
 
 // The gather's stream of a plan: the pieces (w = 0) or the pieces cut into
 // chunks of at most w final columns, ordered chunk after chunk down each
-// column interval (w > 0), in that order (G2G_ORDER_UNION) or ordered by
-// destination address (G2G_ORDER_DESTINATION). Built on first use, rebuilt
-// for another (w, order). Chunks: vertically adjacent pieces then move the
-// same columns at about the same time: their destination segments, contiguous
-// within a final block column, are written together (longer DRAM runs, and the
-// cache line two pieces share when a segment does not end on a 128-byte line
-// is merged in L2 instead of reaching memory partly written, twice).
-// Destination order: work items that run at the same time write neighbouring
-// destination addresses, final block after final block.
-int select_stream(g2g_plan p, uint32_t w, uint32_t order, hipStream_t s) {
-    if ((w == 0 && order == G2G_ORDER_UNION) || p->fused.n == 0) {
+// column interval (w > 0), untransposed streams optionally cut again into
+// chunks of at most h rows (h > 0, each piece's row chunks in row order), in
+// that order (G2G_ORDER_UNION) or ordered by destination address
+// (G2G_ORDER_DESTINATION). Built on first use, rebuilt for another (w, h,
+// order). Column chunks: vertically adjacent pieces then move the same
+// columns at about the same time: their destination segments, contiguous
+// within a final block column, are written together (longer DRAM runs, and
+// the cache line two pieces share when a segment does not end on a 128-byte
+// line is merged in L2 instead of reaching memory partly written, twice).
+// Row chunks: a tall piece's long column runs become runs of h rows, so that
+// a work item moves a few-KiB-high band of many columns at a large stride
+// rather than a few long contiguous runs. Destination order: work items that
+// run at the same time write neighbouring destination addresses, final block
+// after final block.
+int select_stream(g2g_plan p, uint32_t w, uint32_t order, uint32_t h, hipStream_t s) {
+    if ((w == 0 && h == 0 && order == G2G_ORDER_UNION) || p->fused.n == 0) {
         if (p->use_alt) ++p->stream_gen;
         p->use_alt = false;
         return G2G_OK;
     }
-    if (p->alt.n && p->alt_w == w && p->alt_order == order) {
+    if (h && p->trans != 'N') return fail(G2G_ERR_INVALID, "row chunks apply to untransposed plans only");
+    if (p->alt.n && p->alt_w == w && p->alt_order == order && p->alt_h == h) {
         if (!p->use_alt) ++p->stream_gen;
         p->use_alt = true;
         return G2G_OK;
@@ -752,10 +764,15 @@ int select_stream(g2g_plan p, uint32_t w, uint32_t order, hipStream_t s) {
     p->alt.tile_bytes = 0;
     p->alt_w = w;
     p->alt_order = order;
-    // the stream's pieces, before ordering: chunks of w columns, or the pieces
-    dbuf<piece> base;
-    dbuf<uint64_t> base_bytes;
+    p->alt_h = h;
+    // the stream's pieces before ordering: the pieces, column chunks of
+    // them, row chunks of either; `bytes` of a built stage
+    const piece *cur = p->fused.pieces.p;
     int64_t n = p->fused.n;
+    dbuf<piece> cbuf, rbuf;
+    dbuf<uint64_t> cbytes, rbytes;
+    dbuf<piece> *last = nullptr;
+    dbuf<uint64_t> *last_bytes = nullptr;
     if (w) {
         const int64_t n_ur = (int64_t)p->urows.size();
         std::vector<uint32_t> cuj, cc0;
@@ -774,21 +791,42 @@ int select_stream(g2g_plan p, uint32_t w, uint32_t order, hipStream_t s) {
         dbuf<uint32_t> d_cuj, d_cc0;
         G2G_TRY(d_cuj.upload(cuj.data(), cuj.size(), s));
         G2G_TRY(d_cc0.upload(cc0.data(), cc0.size(), s));
-        G2G_TRY(base.alloc((size_t)n));
-        G2G_TRY(base_bytes.alloc((size_t)n));
+        G2G_TRY(cbuf.alloc((size_t)n));
+        G2G_TRY(cbytes.alloc((size_t)n));
         G2G_HIP(launch_chunk_pieces(p->fused.pieces.p, (int32_t)n_ur, d_cuj.p, d_cc0.p, n_chunks, w, p->es,
-                                    p->trans != 'N', base.p, base_bytes.p, s));
+                                    p->trans != 'N', cbuf.p, cbytes.p, s));
         G2G_HIP(hipStreamSynchronize(s));  // the chunk tables go out of scope
+        cur = cbuf.p;
+        last = &cbuf;
+        last_bytes = &cbytes;
     }
-    const piece *src = w ? base.p : p->fused.pieces.p;
-    G2G_TRY(p->alt.pieces.alloc((size_t)n));
-    G2G_TRY(p->alt.pend.alloc((size_t)n));
+    if (h) {
+        dbuf<uint64_t> ends, scratch;
+        G2G_TRY(ends.alloc((size_t)n));
+        G2G_HIP(launch_row_counts(cur, n, h, ends.p, s));
+        G2G_TRY(scratch.alloc(scan_scratch_elems(n)));
+        G2G_HIP(inclusive_scan_u64(ends.p, n, scratch.p, s));
+        uint64_t m = 0;
+        G2G_HIP(hipMemcpyAsync(&m, ends.p + n - 1, sizeof m, hipMemcpyDeviceToHost, s));
+        G2G_HIP(hipStreamSynchronize(s));
+        if (m > 0xffffff00ull)
+            return fail(G2G_ERR_UNSUPPORTED, "%llu row chunks of %u rows: too many", (unsigned long long)m, h);
+        G2G_TRY(rbuf.alloc((size_t)m));
+        G2G_TRY(rbytes.alloc((size_t)m));
+        G2G_HIP(launch_row_chunks(cur, n, h, p->es, ends.p, rbuf.p, rbytes.p, s));
+        G2G_HIP(hipStreamSynchronize(s));  // ends goes out of scope
+        cur = rbuf.p;
+        n = (int64_t)m;
+        last = &rbuf;
+        last_bytes = &rbytes;
+    }
     if (order == G2G_ORDER_DESTINATION) {
-        G2G_TRY(sort_pieces_by_dst(src, n, p->es, p->alt.pieces.p, p->alt.pend.p, s));
-    } else {  // chunks in their own order
-        G2G_HIP(hipMemcpyAsync(p->alt.pieces.p, base.p, (size_t)n * sizeof(piece), hipMemcpyDeviceToDevice, s));
-        G2G_HIP(hipMemcpyAsync(p->alt.pend.p, base_bytes.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                               s));
+        G2G_TRY(p->alt.pieces.alloc((size_t)n));
+        G2G_TRY(p->alt.pend.alloc((size_t)n));
+        G2G_TRY(sort_pieces_by_dst(cur, n, p->es, p->alt.pieces.p, p->alt.pend.p, s));
+    } else {  // chunks in their own order: the last stage's tables become the stream's
+        p->alt.pieces.swap(*last);
+        p->alt.pend.swap(*last_bytes);
     }
     dbuf<uint64_t> scratch;
     G2G_TRY(scratch.alloc(scan_scratch_elems(n)));
@@ -878,7 +916,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     p->class_key = shape_class(es, trans, total, np);
     tuned_cfg tc;
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
-        if (tc.chunk_w || tc.order) G2G_TRY(select_stream(p, tc.chunk_w, tc.order, s));
+        if (tc.chunk_w || tc.order || tc.chunk_h) G2G_TRY(select_stream(p, tc.chunk_w, tc.order, tc.chunk_h, s));
         stage_table &gt = gtab(p);
         G2G_TRY(retile(gt, tc.tile_bytes, s));
         gt.cfg = tc.cfg;
@@ -913,6 +951,8 @@ int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
     t->launch_mode = launch_mode_of(gt.cfg);
     t->column_chunks = p->use_alt ? (int32_t)p->alt_w : 0;
     t->stream_order = p->use_alt ? (int32_t)p->alt_order : G2G_ORDER_UNION;
+    t->row_chunks = p->use_alt ? (int32_t)p->alt_h : 0;
+    t->reserved = 0;
     t->ms = p->tuned_ms;
     return G2G_OK;
 }
@@ -926,8 +966,10 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
     if (t->column_chunks < 0) return fail(G2G_ERR_INVALID, "column_chunks %d < 0", (int)t->column_chunks);
     if (t->stream_order != G2G_ORDER_UNION && t->stream_order != G2G_ORDER_DESTINATION)
         return fail(G2G_ERR_INVALID, "unknown stream_order %d", (int)t->stream_order);
+    if (t->row_chunks < 0) return fail(G2G_ERR_INVALID, "row_chunks %d < 0", (int)t->row_chunks);
+    if (t->row_chunks && p->trans != 'N') return fail(G2G_ERR_INVALID, "row chunks apply to untransposed plans only");
     std::lock_guard<std::mutex> lock(p->mu);
-    G2G_TRY(select_stream(p, (uint32_t)t->column_chunks, (uint32_t)t->stream_order, nullptr));
+    G2G_TRY(select_stream(p, (uint32_t)t->column_chunks, (uint32_t)t->stream_order, (uint32_t)t->row_chunks, nullptr));
     stage_table &gt = gtab(p);
     if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
     gt.cfg.threads = t->threads;
@@ -1027,7 +1069,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     // order; the eight fastest configurations of each order are then timed
     // again on it and on column-chunked streams of several widths in the same
     // order (transposed plans: whole groups of 256 bytes of final columns,
-    // the width of a transpose tile of 64 final rows).
+    // the width of a transpose tile of 64 final rows), and on tall pieces
+    // also cut into row chunks.
     std::lock_guard<std::mutex> lock(p->mu);
     if (p->use_alt) ++p->stream_gen;  // batches built on another stream must not run it
     p->use_alt = false;
@@ -1059,13 +1102,23 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     for (uint32_t w : (p->trans == 'N' ? std::vector<uint32_t>{1, 2, 4, 8, 16, 32, 64}
                                        : std::vector<uint32_t>{q, 2 * q, 4 * q}))
         if (chunk_count(p, w) <= (16ull << 20)) widths.push_back(w);
+    // (columns, rows) of each chunked stream timed: the widths above, and for
+    // untransposed plans whose pieces' columns average 2 KiB or more, row
+    // chunks of 1 and 2 KiB on chunks of 8 and 32 columns
+    std::vector<std::pair<uint32_t, uint32_t>> streams;
+    for (uint32_t w : widths) streams.push_back({w, 0u});
+    const uint64_t mean_colb = p->urows.empty() ? 0 : (uint64_t)p->m / p->urows.size() * p->es;
+    if (p->trans == 'N' && mean_colb >= 2048)
+        for (uint32_t hb : {1024u, 2048u})
+            for (uint32_t w : {8u, 32u})
+                if (chunk_count(p, w) * (mean_colb / hb + 1) <= (16ull << 20)) streams.push_back({w, hb / p->es});
     tuned_cfg win{p->fused.tile_bytes, p->fused.cfg, 0};
     double best_ms = 1e30;
     int rc = G2G_OK;
     for (uint32_t order : orders) {
         // sweep every variant, then time the few fastest again with more
         // runs, so that one noisy measurement does not pick the winner
-        if ((rc = select_stream(p, 0, order, s)) != G2G_OK) break;
+        if ((rc = select_stream(p, 0, order, 0, s)) != G2G_OK) break;
         std::vector<tuned_cfg> cand;
         for (uint64_t tb : sizes) {
             for (int th : {256, 512})
@@ -1088,14 +1141,14 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         if (rc != G2G_OK || cand.empty()) break;
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
         const size_t top = std::min<size_t>(8, cand.size());
-        for (uint32_t w : widths) {
-            if ((rc = select_stream(p, w, order, s)) != G2G_OK) break;
+        for (const auto &wh : streams) {
+            if ((rc = select_stream(p, wh.first, order, wh.second, s)) != G2G_OK) break;
             for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
                 double ms = 0;
                 if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, w, order};
+                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, wh.first, order, wh.second};
                 }
             }
             if (rc != G2G_OK) break;
@@ -1103,7 +1156,7 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         if (rc != G2G_OK) break;
     }
     if (rc != G2G_OK) return rc;
-    G2G_TRY(select_stream(p, win.chunk_w, win.order, s));
+    G2G_TRY(select_stream(p, win.chunk_w, win.order, win.chunk_h, s));
     stage_table &gt = gtab(p);
     G2G_TRY(retile(gt, win.tile_bytes, s));
     gt.cfg = win.cfg;
@@ -1541,6 +1594,8 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->launch_mode = launch_mode_of(b->cfg);
         best->column_chunks = 0;
         best->stream_order = G2G_ORDER_UNION;
+        best->row_chunks = 0;
+        best->reserved = 0;
         best->ms = b->tuned_ms;
     }
     return G2G_OK;
@@ -1555,6 +1610,8 @@ int g2g_batch_get_tuning(g2g_batch b, g2g_tuning *t) {
     t->launch_mode = launch_mode_of(b->cfg);
     t->column_chunks = 0;
     t->stream_order = G2G_ORDER_UNION;
+    t->row_chunks = 0;
+    t->reserved = 0;
     t->ms = b->tuned_ms;
     return G2G_OK;
 }
@@ -1565,8 +1622,9 @@ int g2g_batch_set_tuning(g2g_batch b, const g2g_tuning *t, void *stream) {
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) ||
         (t->launch_mode != G2G_LAUNCH_PERSISTENT && t->launch_mode != G2G_LAUNCH_PERSISTENT_XCD))
         return fail(G2G_ERR_INVALID, "batch tuning outside the compiled variants (batches launch persistent)");
-    if (t->column_chunks != 0 || t->stream_order != G2G_ORDER_UNION)
-        return fail(G2G_ERR_INVALID, "a batch runs each plan's own stream: column_chunks and stream_order must be 0");
+    if (t->column_chunks != 0 || t->stream_order != G2G_ORDER_UNION || t->row_chunks != 0)
+        return fail(G2G_ERR_INVALID,
+                    "a batch runs each plan's own stream: column_chunks, stream_order and row_chunks must be 0");
     if ((uint64_t)t->tile_bytes != b->tile_bytes) G2G_TRY(batch_retile(b, (uint64_t)t->tile_bytes, (hipStream_t)stream));
     b->drop_graph();
     b->cfg.threads = t->threads;

@@ -72,6 +72,17 @@ __host__ __device__ __forceinline__ piece chunk_piece(piece pc, uint32_t c0, uin
     return pc;
 }
 
+// Final rows [r0, r0 + h) of an untransposed piece (fewer at its end): source
+// and destination advance by r0 elements, the columns stay.
+__host__ __device__ __forceinline__ piece row_chunk_piece(piece pc, uint32_t r0, uint32_t h, uint32_t es) {
+    const uint32_t nr = imin(h, pc.rows - r0);
+    pc.src += (uint64_t)r0 * es;
+    pc.dst += (uint64_t)r0 * es;
+    pc.rows = nr;
+    pc.vmul = div_magic((nr * es + 15) / 16);
+    return pc;
+}
+
 // Columns [c0, c1) of a piece that belong to the work item covering stream
 // bytes [lo, hi), for a piece occupying stream bytes [ps, pe): the columns
 // (groups of 256/es columns for transposed pieces) whose first byte lies in

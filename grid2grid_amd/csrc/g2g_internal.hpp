@@ -119,6 +119,13 @@ hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
                                const uint32_t *chunk_c0, int64_t n_chunks, uint32_t w, uint32_t esize,
                                int trans, piece *out, uint64_t *bytes, hipStream_t stream);
 
+// Untransposed pieces cut into chunks of at most h rows: cnt[p] = chunks of
+// piece p; with ends = the inclusive scan of cnt, piece p's chunks go to
+// out[ends[p-1] ..] in row order, bytes[k] = out[k]'s bytes.
+hipError_t launch_row_counts(const piece *in, int64_t n, uint32_t h, uint64_t *cnt, hipStream_t stream);
+hipError_t launch_row_chunks(const piece *in, int64_t n, uint32_t h, uint32_t es, const uint64_t *ends, piece *out,
+                             uint64_t *bytes, hipStream_t stream);
+
 // out = the n pieces of `in` ordered by destination address, bytes[k] =
 // out[k].rows * cols * es. Returns a g2g_status; synchronous.
 int sort_pieces_by_dst(const piece *in, int64_t n, uint32_t es, piece *out, uint64_t *bytes, hipStream_t stream);

@@ -114,6 +114,39 @@ hipError_t launch_chunk_pieces(const piece *pieces, int32_t n_ur, const uint32_t
     return hipGetLastError();
 }
 
+__global__ void k_row_counts(const piece *in, int64_t n, uint32_t h, uint64_t *cnt) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    cnt[p] = (in[p].rows + (uint64_t)h - 1) / h;
+}
+
+hipError_t launch_row_counts(const piece *in, int64_t n, uint32_t h, uint64_t *cnt, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in, n, h, cnt);
+    return hipGetLastError();
+}
+
+__global__ void k_row_chunks(const piece *in, int64_t n, uint32_t h, uint32_t es, const uint64_t *ends, piece *out,
+                             uint64_t *bytes) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const piece pc = in[p];
+    uint64_t o = p ? ends[p - 1] : 0;
+    for (uint32_t r0 = 0; r0 < pc.rows; r0 += h, ++o) {
+        const piece q = row_chunk_piece(pc, r0, h, es);
+        out[o] = q;
+        bytes[o] = (uint64_t)q.rows * q.cols * es;
+    }
+}
+
+hipError_t launch_row_chunks(const piece *in, int64_t n, uint32_t h, uint32_t es, const uint64_t *ends, piece *out,
+                             uint64_t *bytes, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_chunks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in, n, h, es, ends,
+                       out, bytes);
+    return hipGetLastError();
+}
+
 __global__ void k_piece_bytes(const piece *pieces, int64_t n, kreqs kr, uint64_t *bytes) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;

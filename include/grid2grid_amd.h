@@ -36,8 +36,9 @@ extern "C" {
  * 2 adds (nothing removed or changed): g2g_transform_layouts and the plan
  * cache, g2g_transform_graph, g2g_scale_layout, g2g_copy_bytes,
  * g2g_batch_get/set_tuning, g2g_transform_plans_noalloc.
- * 3 adds g2g_tuning.stream_order (the word version 2 reserved as 0) and
- * G2G_ORDER_*. */
+ * 3 changes g2g_tuning: stream_order takes the word version 2 reserved,
+ * and row_chunks plus a reserved word are added before ms (the struct grows
+ * from 40 to 48 bytes); adds G2G_ORDER_*. */
 #define G2G_ABI_VERSION 3
 
 /* Element types; the reference instantiates transform for these four
@@ -366,9 +367,15 @@ typedef struct g2g_tuning {
      * by destination address, so that work items that run at the same time
      * write neighbouring addresses, final block after final block. Batches
      * report and accept only G2G_ORDER_UNION for their own tuning (each
-     * plan's stream runs as the plan is tuned). This word was reserved (0)
-     * before ABI version 3: such callers get the union order. */
+     * plan's stream runs as the plan is tuned). A caller that zeroes the
+     * struct gets the union order. */
     int32_t stream_order;
+    /* 0: whole columns; h > 0 (untransposed plans only): every piece or chunk
+     * of the stream cut again into chunks of at most h rows, so that tall
+     * pieces move as bands of runs of h rows across many columns rather than
+     * as long contiguous runs. Batches: 0. */
+    int32_t row_chunks;
+    int32_t reserved;      /* 0 */
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
 } g2g_tuning;
 

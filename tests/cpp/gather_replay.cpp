@@ -190,7 +190,8 @@ extern "C" {
 // Replays the fused gather of (initial -> final, dtype, trans) with work items
 // of tile_bytes, workgroups of `threads`, kernel feature level `feat`
 // (FEAT_COPY 0, FEAT_TRANS 1, FEAT_ALL 2), column chunks of chunk_w columns
-// in stream order `order` (G2G_ORDER_UNION or G2G_ORDER_DESTINATION)
+// cut again into chunks of chunk_h rows (untransposed; 0: whole columns), in
+// stream order `order` (G2G_ORDER_UNION or G2G_ORDER_DESTINATION)
 // (0: the pieces), scaled = 1 for a scaled request (the scaled path moves
 // every piece). Returns the number of errors; the first ones are described in
 // msg. stats (may be NULL): work items, copy slots, element slots, tile
@@ -203,8 +204,8 @@ extern "C" {
 // axis, 4 a transpose tile's phase 2 one final column off, 5 the LDS row
 // padding dropped in phase 2, 6 work items reaching one column into the next.
 int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype, char trans, int scaled,
-                  int64_t tile_bytes, int threads, int feat, uint32_t chunk_w, int order, int plant, char *msg,
-                  int msg_len,
+                  int64_t tile_bytes, int threads, int feat, uint32_t chunk_w, int order, uint32_t chunk_h,
+                  int plant, char *msg, int msg_len,
                   int64_t *stats) {
     replay R;
     const uint32_t es = dtype == G2G_FLOAT32 ? 4 : dtype == G2G_COMPLEX128 ? 16 : 8;
@@ -239,6 +240,13 @@ int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype
             for (int32_t c0 = 0; c0 < uc[uj].end - uc[uj].start; c0 += (int32_t)chunk_w)
                 for (int64_t ui = 0; ui < n_ur; ++ui)
                     ch.push_back(chunk_piece(pieces[uj * n_ur + ui], (uint32_t)c0, chunk_w, es, R.trans));
+        pieces.swap(ch);
+    }
+    // row chunks of h rows (untransposed streams; k_row_chunks)
+    if (chunk_h && !R.trans) {
+        std::vector<piece> ch;
+        for (const piece &pc : pieces)
+            for (uint32_t r0 = 0; r0 < pc.rows; r0 += chunk_h) ch.push_back(row_chunk_piece(pc, r0, chunk_h, es));
         pieces.swap(ch);
     }
     // destination order (select_stream, sort_pieces_by_dst): a stable sort by

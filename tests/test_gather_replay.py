@@ -26,13 +26,14 @@ DTYPES = [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64, g2g.COMPLEX128]
 _lib = None
 
 
-def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant, order=0):
+def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant, order=0, chunk_h=0):
     global _lib
     if _lib is None:
         _lib = ctypes.CDLL(LIB)
         _lib.replay_gather.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_char, ctypes.c_int,
                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+                                       ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int64)]
     if feat is None:
         feat = FEAT_ALL if scaled else (FEAT_TRANS if trans != "N" else FEAT_COPY)
     es = g2g.DTYPE_SIZE[dtype]
@@ -42,15 +43,17 @@ def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, pl
     msg = ctypes.create_string_buffer(4096)
     stats = (ctypes.c_int64 * 6)()
     errors = _lib.replay_gather(ctypes.addressof(di.struct), ctypes.addressof(df.struct), dtype, trans.encode(),
-                                int(scaled), tile_bytes, threads, feat, chunk_w, order, plant, msg, len(msg), stats)
+                                int(scaled), tile_bytes, threads, feat, chunk_w, order, chunk_h, plant, msg, len(msg),
+                                stats)
     return errors, msg.value.decode(), list(stats)
 
 
 def _replay(init, fin, dtype, trans="N", scaled=False, tile_bytes=65536, threads=512, feat=None, chunk_w=0,
-            order=0):
-    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0, order)
+            order=0, chunk_h=0):
+    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0, order, chunk_h)
     assert errors == 0, (f"{errors} errors ({init.m}x{init.n} dtype {dtype} trans {trans} scaled {scaled} "
-                         f"tile {tile_bytes} threads {threads} feat {feat} chunks {chunk_w} order {order}):\n" + msg)
+                         f"tile {tile_bytes} threads {threads} feat {feat} chunks {chunk_w} order {order} "
+                         f"rows {chunk_h}):\n" + msg)
     return stats
 
 
@@ -69,6 +72,8 @@ def test_replay_untransposed(name):
             _replay(init, fin, dtype, tile_bytes=8192, threads=512, chunk_w=w)
         for w, tb in ((0, 40960), (2, 12288), (16, 49152)):
             _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
+        for w, h, o in ((0, 3, 0), (8, 16, 1), (3, 7, 1)):
+            _replay(init, fin, dtype, tile_bytes=12288, threads=512, chunk_w=w, order=o, chunk_h=h)
 
 
 @pytest.mark.parametrize("name", list(shapes.transposed_pairs()))
@@ -141,6 +146,8 @@ def test_replay_irregular_small(dtype):
     _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=16)
     for w, tb in ((0, 40960), (2, 49152)):
         _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
+    for w, h in ((8, 256), (32, 128)):
+        _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=w, order=1, chunk_h=h)
 
 
 @pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb"),

@@ -286,34 +286,38 @@ def test_column_chunked_stream(name):
     ordered down each column interval) and over the pieces or chunks in
     destination order writes exactly what the pieces in their own order
     write, for odd widths, work-item sizes that are not powers of two and
-    every launch mode; the width and order are reported back; a negative
-    width and an unknown order are refused."""
+    every launch mode, also cut into row chunks; the width, order and row
+    chunk height are reported back; negative widths or heights and an
+    unknown order are refused."""
     init, fin = _layout_pairs()[name]
     dtype = g2g.FLOAT32
     rng = np.random.default_rng(17)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
     want = _expected(init, ia, fin, fa, dtype)
     ti = _dev(ia)
-    for w, mode, tile, order in ((1, 0, 4096, 0), (3, 1, 65536, 1), (16, 2, 4096, 0), (64, 3, 1 << 20, 1),
-                                 (3, 0, 1 << 20, 0), (0, 1, 40960, 1), (0, 0, 4096, 1), (0, 3, 49152, 1),
-                                 (16, 1, 12288, 1)):
-        print(f"chunks {name} w {w} mode {mode} tile {tile} order {order}", flush=True)
+    for w, mode, tile, order, h in ((1, 0, 4096, 0, 0), (3, 1, 65536, 1, 0), (16, 2, 4096, 0, 0),
+                                    (64, 3, 1 << 20, 1, 0), (3, 0, 1 << 20, 0, 0), (0, 1, 40960, 1, 0),
+                                    (0, 0, 4096, 1, 0), (0, 3, 49152, 1, 0), (16, 1, 12288, 1, 0), (0, 1, 4096, 0, 5),
+                                    (8, 0, 12288, 1, 16), (3, 3, 65536, 1, 3), (32, 1, 8192, 0, 64)):
+        print(f"chunks {name} w {w} mode {mode} tile {tile} order {order} rows {h}", flush=True)
         tf = _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
-        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w, stream_order=order)
-        assert plan.tuning["column_chunks"] == w and plan.tuning["stream_order"] == order
+        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        t = plan.tuning
+        assert t["column_chunks"] == w and t["stream_order"] == order and t["row_chunks"] == h
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         # back to the pieces in their own order
         tf.copy_(_dev(fa))
         plan.set_tuning(tile, 256, 64, 1, mode, column_chunks=0)
-        assert plan.tuning["column_chunks"] == 0 and plan.tuning["stream_order"] == 0
+        t = plan.tuning
+        assert t["column_chunks"] == 0 and t["stream_order"] == 0 and t["row_chunks"] == 0
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         plan.close()
-    for bad in ({"column_chunks": -1}, {"stream_order": 2}, {"stream_order": -1}):
+    for bad in ({"column_chunks": -1}, {"stream_order": 2}, {"stream_order": -1}, {"row_chunks": -1}):
         with pytest.raises(g2g.G2GError):
             plan = g2g.Plan(init, ti.data_ptr(), fin, ti.data_ptr(), dtype)
             try:
@@ -343,6 +347,9 @@ def test_column_chunked_transposed(name, dtype):
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
         plan.set_tuning(tile, 512, 128, 1, mode, column_chunks=w, stream_order=order)
         assert plan.tuning["column_chunks"] == w and plan.tuning["stream_order"] == order
+        with pytest.raises(g2g.G2GError):  # row chunks: untransposed plans only
+            plan.set_tuning(tile, 512, 128, 1, mode, column_chunks=w, stream_order=order, row_chunks=16)
+        assert plan.tuning["column_chunks"] == w and plan.tuning["stream_order"] == order
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
@@ -365,6 +372,8 @@ def test_column_chunked_batch_and_autotune():
             plan.set_tuning(65536, 256, 64, 0, 1, column_chunks=2 + k, stream_order=k // 2)
         elif k == 3:
             plan.set_tuning(40960, 512, 128, 0, 1, stream_order=1)
+        elif k == 1:
+            plan.set_tuning(16384, 512, 64, 0, 1, column_chunks=4, row_chunks=6)
         cases.append((plan, ti, tf, fa, _expected(init, ia, fin, fa, dtype)))
     batch = g2g.Batch([c[0] for c in cases])
     batch.execute()

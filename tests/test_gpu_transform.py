@@ -356,6 +356,45 @@ def test_column_chunked_transposed(name, dtype):
         plan.close()
 
 
+def test_tall_pieces_autotune_row_chunks_and_shape_class():
+    """Tall pieces (4 KiB columns: 512-row f64 blocks, and one block per side):
+    the autotuner's pick, which may cut them into row bands, gives the oracle's
+    bytes; a plan of the same shape created afterwards starts from the same
+    stream (column chunks, row chunks, order) and gives them too."""
+    dtype = g2g.FLOAT64
+    rng = np.random.default_rng(61)
+    for init, fin in ((scalapack_layout(2048, 96, 512, 32, 2, 1, "C"), scalapack_layout(2048, 96, 1024, 48, 2, 2, "C")),
+                      (scalapack_layout(1536, 40, 1536, 40, 1, 1, "C"), scalapack_layout(1536, 40, 1536, 40, 1, 1, "C"))):
+        ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+        want = _expected(init, ia, fin, fa, dtype)
+        ti, tf = _dev(ia), _dev(fa)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
+        best = plan.autotune(reps=1)
+        print(f"tall pieces {init.m}x{init.n}: {best}", flush=True)
+        tf.copy_(_dev(fa))
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want)
+        # a new plan of the shape class starts from the recorded stream
+        ti2, tf2 = _dev(ia), _dev(fa)
+        plan2 = g2g.Plan(init, ti2.data_ptr(), fin, tf2.data_ptr(), dtype)
+        t2 = plan2.tuning
+        for k in ("column_chunks", "row_chunks", "stream_order", "tile_bytes"):
+            assert t2[k] == best[k], (k, t2, best)
+        plan2.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf2.cpu().numpy(), want)
+        # and every row-chunk height the tuner does not try, on the same plan
+        for h in (1, 7, 300):
+            tf2.copy_(_dev(fa))
+            plan2.set_tuning(8192, 256, 128, 1, 1, column_chunks=5, stream_order=h % 2, row_chunks=h)
+            plan2.transform()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(tf2.cpu().numpy(), want, err_msg=f"row chunks {h}")
+        plan2.close()
+        plan.close()
+
+
 def test_column_chunked_batch_and_autotune():
     """A batch of chunked and unchunked plans, in union and destination
     order, gives every request's own result; the autotuner's choice (chunked

@@ -412,7 +412,7 @@ int retile(stage_table &t, uint64_t tile_bytes, hipStream_t s) {
     if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
     t.n_tiles = (uint32_t)nt;
     G2G_TRY(t.first.alloc(nt));
-    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, t.first.p, s));
+    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, (int64_t)t.n_tiles, t.first.p, s));
     G2G_HIP(hipStreamSynchronize(s));
     return G2G_OK;
 }
@@ -1393,7 +1393,7 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
         d.tile_bytes = tile_bytes;
         d.n_tiles = (uint32_t)((t.total + tile_bytes - 1) / tile_bytes);
         d.tile_base = (uint32_t)off[k];
-        G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, b->first.p + off[k], s));
+        G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, (int64_t)d.n_tiles, b->first.p + off[k], s));
     }
     G2G_TRY(b->descs.upload(b->host_descs.data(), b->host_descs.size(), s));
     G2G_HIP(launch_work_records(b->descs.p, (int32_t)b->host_descs.size(), b->work.p, s));

@@ -139,7 +139,7 @@ size_t scan_scratch_elems(int64_t n);
 hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipStream_t stream);
 
 // first_piece[t] = the piece containing byte t*tile_bytes of the piece stream.
-hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes,
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, int64_t n_tiles,
                               uint32_t *first_piece, hipStream_t stream);
 
 // Wire order of the plain form: per piece, the element offset inside its

@@ -239,19 +239,30 @@ hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipS
     return hipGetLastError();
 }
 
-__global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, uint32_t *fp) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint64_t s = p ? pend[p - 1] : 0, e = pend[p];
-    if (e <= s) return;
-    for (uint64_t t = (s + tb - 1) / tb; t * tb < e; ++t) fp[t] = (uint32_t)p;
+// first_piece[t] = the piece holding byte t*tb of the stream: the first p
+// with pend[p] > t*tb (pend is the inclusive byte scan, so empty pieces are
+// skipped). One thread per work item, a binary search over the scan: the
+// same work for one huge piece as for many small ones.
+__global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, int64_t n_tiles, uint32_t *fp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const uint64_t b = (uint64_t)t * tb;
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (pend[mid] > b)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    fp[t] = (uint32_t)lo;
 }
 
-hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes,
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, int64_t n_tiles,
                               uint32_t *first_piece, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_first_piece, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pend, n,
-                       tile_bytes, first_piece);
+    if (n == 0 || n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_first_piece, dim3((unsigned)((n_tiles + 255) / 256)), dim3(256), 0, stream, pend, n,
+                       tile_bytes, n_tiles, first_piece);
     return hipGetLastError();
 }
 

@@ -73,3 +73,39 @@ def irregular_small():
     init = arbitrary_layout(rng, 6000, 400, 16, 5, 6, lo=40, hi=520, pad=(0, 63), gap=(0, 64))
     fin = arbitrary_layout(rng, 6000, 400, 13, 4, 5, lo=40, hi=520, pad=(0, 63), gap=(0, 64))
     return init, fin
+
+
+def random_stream_cases(seed=4242, count=36):
+    """Random requests (block-cyclic from the mixed-request distribution and
+    irregular arbitrary layouts, every type, 'N'/'T'/'C') with random
+    tunings: (init, final, dtype, trans, tuning), tuning = (tile_bytes,
+    threads, unroll_bytes, nontemporal, launch_mode, column_chunks,
+    stream_order, row_chunks)."""
+    from grid2grid_amd import workloads
+    from grid2grid_amd import _lib
+    dtypes = [_lib.FLOAT32, _lib.FLOAT64, _lib.COMPLEX64, _lib.COMPLEX128]
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(count):
+        if k % 3 == 2:
+            m, n = (int(x) for x in rng.integers(20, 400, size=2))
+            p1, p2 = (int(x) for x in rng.integers(1, 9, size=2))
+            init = arbitrary_layout(rng, m, n, int(rng.integers(1, 8)), int(rng.integers(1, 8)), p1, lo=3, hi=90,
+                                    pad=(0, 9), gap=(0, 17))
+            trans = "N" if rng.random() < 0.6 else "T"
+            fm, fn = (n, m) if trans == "T" else (m, n)
+            fin = arbitrary_layout(rng, fm, fn, int(rng.integers(1, 8)), int(rng.integers(1, 8)), p2, lo=3, hi=90,
+                                   pad=(0, 9), gap=(0, 17))
+            dtype = dtypes[int(rng.integers(0, 4))]
+        else:
+            init, fin, dtype, trans = workloads.random_request(rng, dtypes=dtypes, lo=8, hi=400, p_trans=0.4)
+        if trans == "T" and dtype in (_lib.COMPLEX64, _lib.COMPLEX128) and rng.random() < 0.5:
+            trans = "C"
+        tile = int(rng.choice([4096, 12288, 40960, 49152, 65536, 98304, 1 << 20]))
+        th, ub, nt, mode = (int(rng.choice([256, 512])), int(rng.choice([64, 128])), int(rng.integers(0, 2)),
+                            int(rng.integers(0, 4)))
+        w = int(rng.choice([0, 1, 2, 3, 8, 16, 64]))
+        order = int(rng.integers(0, 2))
+        h = int(rng.choice([0, 0, 1, 5, 16, 128])) if trans == "N" else 0
+        cases.append((init, fin, dtype, trans, (tile, th, ub, nt, mode, w, order, h)))
+    return cases

@@ -150,6 +150,18 @@ def test_replay_irregular_small(dtype):
         _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=w, order=1, chunk_h=h)
 
 
+def test_replay_random_streams():
+    """The random requests and tunings of the GPU test
+    test_random_requests_random_streams, replayed: every work item, queue
+    slot and transpose tile of each stream (column and row chunks, either
+    order, any work-item size) checked against the transform's definition."""
+    feats = {"N": FEAT_COPY, "T": FEAT_TRANS, "C": FEAT_TRANS}
+    for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases()):
+        tile, th, ub, nt, mode, w, order, h = tu
+        _replay(init, fin, dtype, trans, tile_bytes=tile, threads=th, feat=feats[trans], chunk_w=w, order=order,
+                chunk_h=h)
+
+
 @pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb"),
                                         (1, "irr"), (2, "irr"), (6, "irr")])
 def test_replay_catches_planted_mistakes(plant, case):

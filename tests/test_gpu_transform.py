@@ -488,6 +488,39 @@ def test_random_requests_both_forms():
         np.testing.assert_array_equal(got, want, err_msg=f"case {k}")
 
 
+def test_random_requests_random_streams():
+    """Random requests (block-cyclic and irregular layouts, every type,
+    'N'/'T'/'C') under random tunings: work-item sizes that are not powers of
+    two, both workgroup sizes and unrolls, every launch mode, column chunks,
+    row chunks (untransposed), union or destination order, each transform
+    against the oracle on random bytes; then the same plans in one batch.
+    The host replay runs the same cases (tests/test_gather_replay.py)."""
+    rng = np.random.default_rng(4343)
+    cases = []
+    for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases()):
+        tile, th, ub, nt, mode, w, order, h = tu
+        ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
+        print(f"case {k}: {init.m}x{init.n} dtype {dtype} trans {trans} tuning {tu}", flush=True)
+        want = _expected(init, ia, fin, fa, dtype, trans)
+        ti, tf = _dev(ia), _dev(fa)
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        plan.transform()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tf.cpu().numpy(), want, err_msg=f"case {k}")
+        cases.append((plan, ti, tf, fa, want))
+    for plan, ti, tf, fa, want in cases:
+        tf.copy_(_dev(fa))
+    batch = g2g.Batch([c[0] for c in cases])
+    batch.execute()
+    torch.cuda.synchronize()
+    for k, (plan, ti, tf, fa, want) in enumerate(cases):
+        np.testing.assert_array_equal(tf.cpu().numpy(), want, err_msg=f"batched case {k}")
+    batch.close()
+    for c in cases:
+        c[0].close()
+
+
 def test_beyond_32_bit_element_indices():
     """65536 x 40000 f32 (2.6e9 elements, 10.5 GB per side): element offsets,
     piece addresses and the seeded values' indices all pass 2^31 and 2^32.

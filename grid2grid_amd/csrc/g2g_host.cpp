@@ -1066,7 +1066,8 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     kr.r[0] = make_request(p->dtype, p->trans, nullptr, nullptr, true, true);
     kr.r[1] = kr.r[0];
     // The sweep runs over the pieces, in their own order and in destination
-    // order; the eight fastest configurations of each order are then timed
+    // order; the eight fastest configurations of each order, at their own and
+    // the neighbouring work-item sizes, are then timed
     // again on it and on column-chunked streams of several widths in the same
     // order (transposed plans: whole groups of 256 bytes of final columns,
     // the width of a transpose tile of 64 final rows), and on tall pieces
@@ -1140,15 +1141,30 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         }
         if (rc != G2G_OK || cand.empty()) break;
         std::sort(cand.begin(), cand.end(), [](const tuned_cfg &x, const tuned_cfg &y) { return x.ms < y.ms; });
-        const size_t top = std::min<size_t>(8, cand.size());
+        // the eight fastest, each also at the neighbouring work-item sizes
+        // (a chunked stream's best size can be one step from the pieces')
+        std::vector<tuned_cfg> retime;
+        for (size_t k = 0; k < std::min<size_t>(8, cand.size()); ++k) {
+            const size_t i = (size_t)(std::find(sizes.begin(), sizes.end(), cand[k].tile_bytes) - sizes.begin());
+            for (size_t j = (i ? i - 1 : i); j <= std::min(i + 1, sizes.size() - 1); ++j) {
+                tuned_cfg c = cand[k];
+                c.tile_bytes = sizes[j];
+                bool dup = false;
+                for (const tuned_cfg &r : retime)
+                    dup = dup || (r.tile_bytes == c.tile_bytes && r.cfg.threads == c.cfg.threads &&
+                                  r.cfg.unroll_bytes == c.cfg.unroll_bytes && r.cfg.nontemporal == c.cfg.nontemporal &&
+                                  r.cfg.persistent == c.cfg.persistent && r.cfg.xcd_ranges == c.cfg.xcd_ranges);
+                if (!dup) retime.push_back(c);
+            }
+        }
         for (const auto &wh : streams) {
             if ((rc = select_stream(p, wh.first, order, wh.second, s)) != G2G_OK) break;
-            for (size_t k = 0; k < top && rc == G2G_OK; ++k) {
+            for (size_t k = 0; k < retime.size() && rc == G2G_OK; ++k) {
                 double ms = 0;
-                if ((rc = time_cfg(cand[k].tile_bytes, cand[k].cfg, 3 * reps, ms)) != G2G_OK) break;
+                if ((rc = time_cfg(retime[k].tile_bytes, retime[k].cfg, 3 * reps, ms)) != G2G_OK) break;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    win = tuned_cfg{cand[k].tile_bytes, cand[k].cfg, ms, wh.first, order, wh.second};
+                    win = tuned_cfg{retime[k].tile_bytes, retime[k].cfg, ms, wh.first, order, wh.second};
                 }
             }
             if (rc != G2G_OK) break;

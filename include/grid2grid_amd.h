@@ -385,8 +385,9 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
 /* Autotuner: time every compiled variant (workgroup size, bytes in flight,
  * non-temporal hint, launch mode) at several work-item sizes on this plan's
  * pieces, in union-cell and in destination order (`reps` timed runs each,
- * on `stream`, hipEvents), time the eight fastest of each order again on the
- * pieces and on column-chunked streams of several widths in the same order,
+ * on `stream`, hipEvents), time the eight fastest of each order again (each
+ * also at the neighbouring work-item sizes) on the pieces and on
+ * column-chunked (and, for tall pieces, row-chunked) streams in that order,
  * keep the fastest for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform

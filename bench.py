@@ -41,7 +41,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed steps first: back-to-back launches start slower for about 15 ms, then settle")
     ap.add_argument("--workload", default="bc32768",
                     help="bc4096 | bc32768 | irregular | copy | bc:m:n:imb:inb:ipr:ipc:fmb:fnb:fpr:fpc:dtype:trans")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")

@@ -1568,7 +1568,8 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
     std::vector<std::pair<uint64_t, std::pair<launch_cfg, double>>> cand;
     int rc = G2G_OK;
     if (b->tile_bytes != 32768) rc = batch_retile(b, 32768, s);
-    for (uint64_t tb = 32768; tb <= (1u << 20) && rc == G2G_OK; tb *= 2)
+    for (uint64_t tb : {32768u, 40960u, 49152u, 65536u, 98304u, 131072u, 262144u, 524288u, 1048576u}) {
+        if (rc != G2G_OK) break;
         for (int th : {256, 512})
             for (int ub : {64, 128})
                 for (int nt = 0; nt < 2 && rc == G2G_OK; ++nt)
@@ -1582,6 +1583,7 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
                         if ((rc = time_cfg(tb, c, reps, ms)) != G2G_OK) break;
                         cand.push_back({tb, {c, ms}});
                     }
+    }
     double best_ms = 1e30;
     uint64_t best_tb = b->tile_bytes;
     launch_cfg best_cfg = b->cfg;

@@ -244,8 +244,9 @@ int g2g_batch_execute_graph(g2g_batch batch, void *stream);
 struct g2g_tuning;
 int g2g_batch_autotune(g2g_batch batch, void *stream, int reps, struct g2g_tuning *best);
 
-/* Launch configuration of a batch (column_chunks is 0: each plan runs its
- * own stream) and setting one: launch_mode G2G_LAUNCH_PERSISTENT or
+/* Launch configuration of a batch (column_chunks, stream_order and
+ * row_chunks are 0: each plan runs its own stream) and setting one (those
+ * three must be 0): launch_mode G2G_LAUNCH_PERSISTENT or
  * G2G_LAUNCH_PERSISTENT_XCD; a new work-item size re-cuts the batch's work
  * items (synchronous on `stream`). The captured graph, if any, is dropped. */
 int g2g_batch_get_tuning(g2g_batch batch, struct g2g_tuning *t);

@@ -1,13 +1,15 @@
 """GPU: g2g_transform_plans_noalloc -- several plans in one launch with the
 launch's descriptors, work records and counters in the library's static
 device array -- gives the bytes of the existing entry point it varies
-(g2g_batch_create + g2g_batch_execute) and of the oracle, on two calls in a
-row, the second with fewer and smaller transforms; requests beyond the
-array's capacity are refused before anything runs."""
+(g2g_batch_create + g2g_batch_execute; for plans on tuned streams, each
+plan's g2g_transform) and of the oracle, on two calls in a row, the second
+with fewer and smaller transforms; requests beyond the array's capacity are
+refused before anything runs."""
 import numpy as np
 import pytest
 
 import oracle_lib as O
+import shapes
 import grid2grid_amd as g2g
 from grid2grid_amd import workloads
 
@@ -102,3 +104,44 @@ def test_noalloc_refuses_beyond_capacity():
     assert bool((tf == 7).all())
     big.close()
     cases[0]["plan"].close()
+
+
+def test_noalloc_tuned_streams_two_calls():
+    """Plans on tuned streams (column chunks, row chunks, destination order,
+    work-item sizes that are not powers of two) run through the allocation-free
+    launch give the bytes of their own g2g_transform and of the oracle; the
+    second call takes fewer plans with smaller inputs."""
+    rng = np.random.default_rng(77)
+    all_cases = []
+    for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases(seed=5151, count=18)):
+        tile, th, ub, nt, mode, w, order, h = tu
+        es = g2g.DTYPE_SIZE[dtype]
+        ia = rng.integers(0, 256, size=init.arena_elems * es, dtype=np.uint8)
+        fa = rng.integers(0, 256, size=fin.arena_elems * es, dtype=np.uint8)
+        want = fa.copy()
+        O.transform(init, ia.copy(), fin, want, dtype, trans, None, None)
+        ti, tf = torch.from_numpy(ia).cuda(), torch.from_numpy(fa).cuda()
+        plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
+        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        all_cases.append(dict(plan=plan, ti=ti, tf=tf, fa=fa, want=want, alpha=None, beta=None,
+                              size=init.m * init.n * es, tuning=tu))
+    by_size = sorted(all_cases, key=lambda c: c["size"])
+    first = all_cases
+    second = by_size[:6]  # fewer and the smallest: the array keeps the first call's contents
+    s = torch.cuda.Stream()
+    for cases in (first, second):
+        _reset(cases)
+        for c in cases:
+            c["plan"].transform()
+        torch.cuda.synchronize()
+        own = [c["tf"].cpu().numpy() for c in cases]
+        _reset(cases)
+        with torch.cuda.stream(s):
+            _call(cases, s)
+        s.synchronize()
+        for k, c in enumerate(cases):
+            got = c["tf"].cpu().numpy()
+            np.testing.assert_array_equal(got, own[k], err_msg=f"request {k} {c['tuning']}: noalloc vs transform")
+            np.testing.assert_array_equal(got, c["want"], err_msg=f"request {k} {c['tuning']}: noalloc vs oracle")
+    for c in all_cases:
+        c["plan"].close()

@@ -107,7 +107,8 @@ def time_steps(torch, fn, steps, warmup, stream, per_launch=False):
     """Average ms per step over `steps` back-to-back steps (events only at the
     two ends of the timed region). With per_launch, a second, separate pass
     brackets every step with its own event pair on the launch stream and
-    returns those durations too (the kernel's average launch duration)."""
+    returns those durations too (their spread; each also holds the dispatch
+    latency between the first event and the kernel's start)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -142,6 +143,8 @@ def read_traffic(workload, mode):
 
 
 def roofline(nbytes, per_launch_ms, workload, mode="fused"):
+    """per_launch_ms: the average launch duration over the timed region (one
+    launch per step, events at the region's two ends)."""
     achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(workload, mode),
@@ -212,7 +215,7 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
                                                                          "nontemporal", "launch_mode",
                                                                          "stream_order")),
         "launch_ms": launch_stats(per),
-        "roofline": roofline(nbytes, statistics.mean(per), name),
+        "roofline": roofline(nbytes, ms, name),
     }
     log(f"leg {name}: {ms:.4f} ms/step, {out['value']} GB/s, frac {out['roofline']['frac']}, mismatches {bad}")
     if facade_calls:
@@ -604,8 +607,7 @@ def main():
     bad = g2g.verify_layout(fin, tf.data_ptr(), dtype, 2, trans, a_n_cols, stream)
     nbytes = workloads.algorithmic_bytes(fin, dtype)
     gbs = nbytes / (ms * 1e-3) / 1e9
-    per_launch_ms = statistics.mean(per)
-    achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
+    achieved = gbs  # one launch per step: the average launch duration over the timed region is ms
     log(f"{args.mode}: {ms:.3f} ms/step, {gbs:.1f} GB/s, mismatches {bad}")
 
     result = {
@@ -635,10 +637,13 @@ def main():
         },
         "bit_exact": bad == 0,
         "mismatches": int(bad),
-        # every launch of the second pass timed on its own (HIP events on the
-        # launch stream): the spread behind roofline.achieved, which uses the mean
+        # every launch of a second pass timed on its own (an event pair on the
+        # launch stream around each): the spread of the launch durations.
+        # roofline.achieved uses the timed region's average (events at its two
+        # ends): an event pair around one launch adds the dispatch latency,
+        # about 2-3 us, which is 5 % of a 50 us launch
         "launch_ms": launch_stats(per),
-        "roofline": roofline(nbytes, per_launch_ms, args.workload, args.mode),
+        "roofline": roofline(nbytes, ms, args.workload, args.mode),
     }
 
     # streaming reference rates of this device on the same buffers (16-byte

@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (default configuration)")
     ap.add_argument("--tuning", default=None,
                     help="explicit configuration tile_bytes:threads:unroll_bytes:nontemporal:launch_mode"
-                         "[:column_chunks[:stream_order[:row_chunks]]] (skips the autotuner)")
+                         "[:column_chunks[:stream_order[:row_chunks[:tail_split]]]] (skips the autotuner)")
     ap.add_argument("--mode", default="fused", choices=["fused", "staged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -161,7 +161,7 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
     own plan, run round-robin (one step = one transform of one pair), so that
     a working set below the 256 MB Infinity Cache is still read from HBM.
     The first plan is autotuned (or set to `tuning`, a tile:threads:unroll:
-    nontemporal:mode[:chunks] string); the others start from the tuning it
+    nontemporal:mode[:chunks[:order[:rows[:tail_split]]]] string); the others start from the tuning it
     records for their shape. With facade_calls, the drop-in call
     (transform_layouts) is timed on the same pairs too."""
     init, fin, dtype, trans = build_workload(name)
@@ -176,7 +176,8 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         if tuning is not None:
             f = [int(x) for x in tuning.split(":")]
             plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
-                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0)
+                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0,
+                            tail_split=f[8] if len(f) > 8 else 0)
         elif k == 0:
             plan.autotune(stream=stream, reps=3)
         pairs.append((ti, tf, plan))
@@ -210,10 +211,10 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
         "pieces": int(info.n_pieces),
         "work_items": int(info.n_tiles),
         "tuning": {k: tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                          "column_chunks", "stream_order", "row_chunks")},
+                                          "column_chunks", "stream_order", "row_chunks", "tail_split")},
         "later_pairs_start_tuned": all(later[k] == tuning[k] for k in ("tile_bytes", "threads", "unroll_bytes",
                                                                          "nontemporal", "launch_mode",
-                                                                         "stream_order")),
+                                                                         "stream_order", "tail_split")),
         "launch_ms": launch_stats(per),
         "roofline": roofline(nbytes, ms, name),
     }
@@ -591,7 +592,8 @@ def main():
     if args.tuning:
         f = [int(x) for x in args.tuning.split(":")]
         plan.set_tuning(*f[:5], column_chunks=f[5] if len(f) > 5 else 0,
-                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0)
+                            stream_order=f[6] if len(f) > 6 else 0, row_chunks=f[7] if len(f) > 7 else 0,
+                            tail_split=f[8] if len(f) > 8 else 0)
         tuning = plan.tuning
         info = plan.info
     elif args.mode == "fused" and not args.no_autotune:
@@ -632,7 +634,8 @@ def main():
             "work_items": int(info.n_tiles),
             "tile_bytes": int(info.tile_bytes),
             "kernel_variant": {k: tuning[k] for k in ("threads", "unroll_bytes", "nontemporal", "launch_mode",
-                                                      "column_chunks", "stream_order", "row_chunks")},
+                                                      "column_chunks", "stream_order", "row_chunks",
+                                                      "tail_split")},
             "parallelism": "1 GPU, P virtual ranks",
         },
         "bit_exact": bad == 0,

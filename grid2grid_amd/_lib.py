@@ -50,7 +50,7 @@ class G2GTuning(ctypes.Structure):
     _fields_ = [("tile_bytes", ctypes.c_int64), ("threads", ctypes.c_int32),
                 ("unroll_bytes", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
                 ("launch_mode", ctypes.c_int32), ("column_chunks", ctypes.c_int32),
-                ("stream_order", ctypes.c_int32), ("row_chunks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("stream_order", ctypes.c_int32), ("row_chunks", ctypes.c_int32), ("tail_split", ctypes.c_int32),
                 ("ms", ctypes.c_double)]
 
 

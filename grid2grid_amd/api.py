@@ -28,7 +28,7 @@ def _scalar_bytes(dtype, value):
 
 
 def _tuning_dict(t):
-    return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
+    return {k: getattr(t, k) for k, _ in t._fields_}
 
 
 def _stream(s):
@@ -121,16 +121,17 @@ class Plan:
         return _tuning_dict(t)
 
     def set_tuning(self, tile_bytes, threads=256, unroll_bytes=64, nontemporal=0, launch_mode=0,
-                   column_chunks=0, stream_order=0, row_chunks=0):
+                   column_chunks=0, stream_order=0, row_chunks=0, tail_split=0):
         """launch_mode: 0 one workgroup per work item, 1 persistent grid, 2 one
         workgroup per work item with a contiguous range of work items per XCD,
         3 persistent with one counter per XCD. column_chunks: 0 runs the
         pieces, w > 0 chunks of at most w columns ordered down each column
         interval. stream_order: 0 union-cell order, 1 destination order.
         row_chunks: 0 whole columns, h > 0 (untransposed plans) chunks of at
-        most h rows."""
+        most h rows. tail_split: 0, or 2/4/8: the stream's last round of work
+        items cut into items of tile_bytes / tail_split."""
         t = _lib.G2GTuning(int(tile_bytes), threads, unroll_bytes, int(nontemporal), int(launch_mode),
-                           int(column_chunks), int(stream_order), int(row_chunks), 0, 0.0)
+                           int(column_chunks), int(stream_order), int(row_chunks), int(tail_split), 0.0)
         check(_lib.load().g2g_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, stream=None, reps=3):

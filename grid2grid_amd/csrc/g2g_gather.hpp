@@ -894,8 +894,9 @@ __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t lane) {
     return x;
 }
 
-// Work item t covers bytes [t*tile_bytes, (t+1)*tile_bytes) of the piece
-// stream (pieces concatenated in table order, each column-major). It moves
+// Work item t covers bytes [lo, hi) of the piece stream (pieces concatenated
+// in table order, each column-major; item_range: tile_bytes-long items, the
+// last ones of a split stream shorter). It moves
 // every column (every group of 256/sizeof(T) columns for transposed pieces)
 // whose first byte lies in that range, so work items partition the columns
 // exactly. Pieces are examined 64 at a time, one per lane (every wave does the
@@ -1006,8 +1007,7 @@ __device__ __forceinline__ void process_work(const gather_args &a, uint32_t w, u
     if (!a.descs) {
         job.pieces = a.one.pieces;
         job.pend = a.one.pend;
-        job.lo = (uint64_t)w * a.one.tile_bytes;
-        job.hi = min(job.lo + a.one.tile_bytes, a.one.total_bytes);
+        item_range(a.one.tile_bytes, a.one.split_from, a.one.split_bytes, a.one.total_bytes, w, job.lo, job.hi);
         job.first_piece = a.one.first_piece[w];
         job.n_pieces = a.one.n_pieces;
         job.desc = 0;

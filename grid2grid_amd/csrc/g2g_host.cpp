@@ -318,6 +318,9 @@ struct stage_table {
     uint64_t total = 0;
     uint64_t tile_bytes = 0;
     uint32_t n_tiles = 0;
+    uint32_t split = 0;        // 0, or the divisor of the last round's work items (retile)
+    uint32_t split_from = 0;   // item_range
+    uint32_t split_bytes = 0;
     launch_cfg cfg;
 };
 
@@ -406,13 +409,29 @@ int launch_mode_of(const launch_cfg &c) {
 int grid_limit() { return device_cus() * 8; }
 
 // Work items of a table for a given size: first piece per item.
-int retile(stage_table &t, uint64_t tile_bytes, hipStream_t s) {
-    t.tile_bytes = tile_bytes;
-    const uint64_t nt = (t.total + t.tile_bytes - 1) / t.tile_bytes;
+// Work items of a stream: tile_bytes each; with split = 2, 4 or 8, the last
+// round of a launch -- the stream's last min(total / 2, CUs x 4 x tile_bytes)
+// bytes, the items before them whole -- in items of tile_bytes / split, so
+// that the workgroups of a launch finish closer together.
+int retile(stage_table &t, uint64_t tile_bytes, hipStream_t s, uint32_t split = 0) {
+    if (split != 0 && split != 2 && split != 4 && split != 8)
+        return fail(G2G_ERR_INVALID, "tail_split %u: 0, 2, 4 or 8", split);
+    uint64_t full = (t.total + tile_bytes - 1) / tile_bytes, nt = full, sb = tile_bytes;
+    if (split) {
+        sb = tile_bytes / split;
+        const uint64_t tail = std::min(t.total / 2, (uint64_t)device_cus() * 4 * tile_bytes);
+        full = (t.total - tail) / tile_bytes;
+        nt = full + (t.total - full * tile_bytes + sb - 1) / sb;
+    }
     if (nt > 0xffffffffull) return fail(G2G_ERR_UNSUPPORTED, "too many work items");
+    t.tile_bytes = tile_bytes;
+    t.split = split;
     t.n_tiles = (uint32_t)nt;
+    t.split_from = (uint32_t)full;
+    t.split_bytes = (uint32_t)sb;
     G2G_TRY(t.first.alloc(nt));
-    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, (int64_t)t.n_tiles, t.first.p, s));
+    G2G_HIP(launch_first_piece(t.pend.p, t.n, t.tile_bytes, t.split_from, t.split_bytes, (int64_t)t.n_tiles, t.first.p,
+                               s));
     G2G_HIP(hipStreamSynchronize(s));
     return G2G_OK;
 }
@@ -425,6 +444,7 @@ struct tuned_cfg {
     uint32_t chunk_w = 0;  // > 0: the gather runs over column chunks of this width
     uint32_t order = 0;    // G2G_ORDER_* of the gather's stream
     uint32_t chunk_h = 0;  // > 0: ... and row chunks of this height
+    uint32_t split = 0;    // retile's split of the last round's work items
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, tuned_cfg>> g_tune_table;
@@ -665,6 +685,8 @@ work_desc make_desc(const stage_table &t, const kreqs &kr, uint32_t tile_base) {
     d.tile_bytes = t.tile_bytes;
     d.total_bytes = t.total;
     d.n_tiles = t.n_tiles;
+    d.split_from = t.split_from;
+    d.split_bytes = t.split_bytes;
     d.tile_base = tile_base;
     d.n_pieces = (uint32_t)t.n;
     d.kr = kr;
@@ -918,7 +940,7 @@ int g2g_plan_create(const g2g_layout *initial, const g2g_layout *final_, int dty
     if (g_tile_bytes.load() == 0 && lookup_tuned(p->class_key, tc)) {
         if (tc.chunk_w || tc.order || tc.chunk_h) G2G_TRY(select_stream(p, tc.chunk_w, tc.order, tc.chunk_h, s));
         stage_table &gt = gtab(p);
-        G2G_TRY(retile(gt, tc.tile_bytes, s));
+        G2G_TRY(retile(gt, tc.tile_bytes, s, tc.split));
         gt.cfg = tc.cfg;
     } else {
         stage_table &gt = gtab(p);
@@ -952,7 +974,7 @@ int g2g_plan_get_tuning(g2g_plan p, g2g_tuning *t) {
     t->column_chunks = p->use_alt ? (int32_t)p->alt_w : 0;
     t->stream_order = p->use_alt ? (int32_t)p->alt_order : G2G_ORDER_UNION;
     t->row_chunks = p->use_alt ? (int32_t)p->alt_h : 0;
-    t->reserved = 0;
+    t->tail_split = (int32_t)gt.split;
     t->ms = p->tuned_ms;
     return G2G_OK;
 }
@@ -968,10 +990,13 @@ int g2g_plan_set_tuning(g2g_plan p, const g2g_tuning *t) {
         return fail(G2G_ERR_INVALID, "unknown stream_order %d", (int)t->stream_order);
     if (t->row_chunks < 0) return fail(G2G_ERR_INVALID, "row_chunks %d < 0", (int)t->row_chunks);
     if (t->row_chunks && p->trans != 'N') return fail(G2G_ERR_INVALID, "row chunks apply to untransposed plans only");
+    if (t->tail_split != 0 && t->tail_split != 2 && t->tail_split != 4 && t->tail_split != 8)
+        return fail(G2G_ERR_INVALID, "tail_split %d: 0, 2, 4 or 8", (int)t->tail_split);
     std::lock_guard<std::mutex> lock(p->mu);
     G2G_TRY(select_stream(p, (uint32_t)t->column_chunks, (uint32_t)t->stream_order, (uint32_t)t->row_chunks, nullptr));
     stage_table &gt = gtab(p);
-    if ((uint64_t)t->tile_bytes != gt.tile_bytes) G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr));
+    if ((uint64_t)t->tile_bytes != gt.tile_bytes || (uint32_t)t->tail_split != gt.split)
+        G2G_TRY(retile(gt, (uint64_t)t->tile_bytes, nullptr, (uint32_t)t->tail_split));
     gt.cfg.threads = t->threads;
     gt.cfg.unroll_bytes = t->unroll_bytes;
     gt.cfg.nontemporal = t->nontemporal != 0;
@@ -1080,9 +1105,9 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
     for (uint64_t tb : {32768u, 40960u, 49152u, 65536u, 98304u, 131072u, 262144u, 524288u, 1048576u})
         if (sizes.empty() || tb < total * 2) sizes.push_back(tb);
     // time one configuration on the current stream (retiled to tb when it differs)
-    auto time_cfg = [&](uint64_t tb, const launch_cfg &c, int n, double &ms_out) -> int {
+    auto time_cfg = [&](uint64_t tb, const launch_cfg &c, int n, double &ms_out, uint32_t split = 0) -> int {
         stage_table &gt = gtab(p);
-        if (gt.tile_bytes != tb) G2G_TRY(retile(gt, tb, s));
+        if (gt.tile_bytes != tb || gt.split != split) G2G_TRY(retile(gt, tb, s, split));
         gt.cfg = c;
         gather_args a;
         std::memset(&a, 0, sizeof a);
@@ -1172,9 +1197,21 @@ int g2g_plan_autotune(g2g_plan p, void *stream, int reps, g2g_tuning *best) {
         if (rc != G2G_OK) break;
     }
     if (rc != G2G_OK) return rc;
+    // the winner timed again as it is and with its last round of work items
+    // split (retile), each the same number of times
     G2G_TRY(select_stream(p, win.chunk_w, win.order, win.chunk_h, s));
+    double split_best = 1e30;
+    for (uint32_t split : {0u, 2u, 4u}) {
+        double ms = 0;
+        G2G_TRY(time_cfg(win.tile_bytes, win.cfg, 3 * reps, ms, split));
+        if (ms < split_best) {
+            split_best = ms;
+            win.ms = ms;
+            win.split = split;
+        }
+    }
     stage_table &gt = gtab(p);
-    G2G_TRY(retile(gt, win.tile_bytes, s));
+    G2G_TRY(retile(gt, win.tile_bytes, s, win.split));
     gt.cfg = win.cfg;
     p->tuned_ms = win.ms;
     store_tuned(p->class_key, win);
@@ -1408,8 +1445,11 @@ int batch_retile(g2g_batch b, uint64_t tile_bytes, hipStream_t s) {
         d.first_piece = b->first.p + off[k];
         d.tile_bytes = tile_bytes;
         d.n_tiles = (uint32_t)((t.total + tile_bytes - 1) / tile_bytes);
+        d.split_from = d.n_tiles;  // a batch's own items are not split
+        d.split_bytes = (uint32_t)tile_bytes;
         d.tile_base = (uint32_t)off[k];
-        G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, (int64_t)d.n_tiles, b->first.p + off[k], s));
+        G2G_HIP(launch_first_piece(t.pend.p, t.n, tile_bytes, d.split_from, d.split_bytes, (int64_t)d.n_tiles,
+                                   b->first.p + off[k], s));
     }
     G2G_TRY(b->descs.upload(b->host_descs.data(), b->host_descs.size(), s));
     G2G_HIP(launch_work_records(b->descs.p, (int32_t)b->host_descs.size(), b->work.p, s));
@@ -1613,7 +1653,7 @@ int g2g_batch_autotune(g2g_batch b, void *stream, int reps, g2g_tuning *best) {
         best->column_chunks = 0;
         best->stream_order = G2G_ORDER_UNION;
         best->row_chunks = 0;
-        best->reserved = 0;
+        best->tail_split = 0;
         best->ms = b->tuned_ms;
     }
     return G2G_OK;
@@ -1629,7 +1669,7 @@ int g2g_batch_get_tuning(g2g_batch b, g2g_tuning *t) {
     t->column_chunks = 0;
     t->stream_order = G2G_ORDER_UNION;
     t->row_chunks = 0;
-    t->reserved = 0;
+    t->tail_split = 0;
     t->ms = b->tuned_ms;
     return G2G_OK;
 }
@@ -1640,9 +1680,10 @@ int g2g_batch_set_tuning(g2g_batch b, const g2g_tuning *t, void *stream) {
         (t->unroll_bytes != 64 && t->unroll_bytes != 128) ||
         (t->launch_mode != G2G_LAUNCH_PERSISTENT && t->launch_mode != G2G_LAUNCH_PERSISTENT_XCD))
         return fail(G2G_ERR_INVALID, "batch tuning outside the compiled variants (batches launch persistent)");
-    if (t->column_chunks != 0 || t->stream_order != G2G_ORDER_UNION || t->row_chunks != 0)
+    if (t->column_chunks != 0 || t->stream_order != G2G_ORDER_UNION || t->row_chunks != 0 || t->tail_split != 0)
         return fail(G2G_ERR_INVALID,
-                    "a batch runs each plan's own stream: column_chunks, stream_order and row_chunks must be 0");
+                    "a batch runs each plan's own stream in its own work items: column_chunks, stream_order, "
+                    "row_chunks and tail_split must be 0");
     if ((uint64_t)t->tile_bytes != b->tile_bytes) G2G_TRY(batch_retile(b, (uint64_t)t->tile_bytes, (hipStream_t)stream));
     b->drop_graph();
     b->cfg.threads = t->threads;

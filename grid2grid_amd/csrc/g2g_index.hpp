@@ -12,6 +12,22 @@ namespace g2g {
 
 __host__ __device__ __forceinline__ uint32_t imin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// Work item t of a piece stream of `total` bytes cut into items of tb bytes,
+// the items from split_from on into shorter ones of split_bytes (the last
+// round of a launch, so that the workgroups finish closer together): its
+// byte range [lo, hi). Unsplit streams have split_from = the item count.
+__host__ __device__ __forceinline__ void item_range(uint64_t tb, uint32_t split_from, uint32_t split_bytes,
+                                                    uint64_t total, uint32_t t, uint64_t &lo, uint64_t &hi) {
+    if (t < split_from) {
+        lo = (uint64_t)t * tb;
+        hi = lo + tb;
+    } else {
+        lo = (uint64_t)split_from * tb + (uint64_t)(t - split_from) * split_bytes;
+        hi = lo + split_bytes;
+    }
+    if (hi > total) hi = total;
+}
+
 __host__ __device__ __forceinline__ uint32_t ceil_log2(uint32_t v) {
     return v <= 1 ? 0u : 32u - (uint32_t)__builtin_clz(v - 1);
 }

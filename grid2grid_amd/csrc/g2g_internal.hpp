@@ -138,9 +138,11 @@ hipError_t launch_piece_bytes(const piece *pieces, int64_t n, const kreqs &kr, u
 size_t scan_scratch_elems(int64_t n);
 hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipStream_t stream);
 
-// first_piece[t] = the piece containing byte t*tile_bytes of the piece stream.
-hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, int64_t n_tiles,
-                              uint32_t *first_piece, hipStream_t stream);
+// first_piece[t] = the piece containing the first byte of work item t of the
+// piece stream (item_range: tile_bytes-long items, from split_from on
+// split_bytes-long ones).
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, uint32_t split_from,
+                              uint32_t split_bytes, int64_t n_tiles, uint32_t *first_piece, hipStream_t stream);
 
 // Wire order of the plain form: per piece, the element offset inside its
 // (source, destination) pair's segment, pieces ordered by position p
@@ -174,9 +176,12 @@ struct work_desc {
     uint32_t n_tiles;
     uint32_t tile_base;           // first global work index of this desc in a launch
     uint32_t n_pieces;
+    uint32_t split_from;          // work items from this one on are split_bytes long (item_range)
+    uint32_t split_bytes;
     uint32_t pad;
     kreqs kr;                     // piece.req selects kr.r[0] or kr.r[1] (same element type)
 };
+static_assert(sizeof(work_desc) == 160, "work_desc layout");
 
 // One work item, everything the gather needs to start on it: the owning
 // transform's piece table and byte scan, the item's byte range [lo, hi) of

@@ -35,8 +35,7 @@ __global__ void k_work_records(const work_desc *descs, work_rec *work) {
         work_rec r;
         r.pieces = d.pieces;
         r.pend = d.pend;
-        r.lo = (uint64_t)t * d.tile_bytes;
-        r.hi = min(r.lo + d.tile_bytes, d.total_bytes);
+        item_range(d.tile_bytes, d.split_from, d.split_bytes, d.total_bytes, t, r.lo, r.hi);
         r.first_piece = d.first_piece[t];
         r.n_pieces = d.n_pieces;
         r.desc = k;
@@ -239,14 +238,16 @@ hipError_t inclusive_scan_u64(uint64_t *data, int64_t n, uint64_t *scratch, hipS
     return hipGetLastError();
 }
 
-// first_piece[t] = the piece holding byte t*tb of the stream: the first p
-// with pend[p] > t*tb (pend is the inclusive byte scan, so empty pieces are
-// skipped). One thread per work item, a binary search over the scan: the
+// first_piece[t] = the piece holding the first byte b of work item t: the
+// first p with pend[p] > b (pend is the inclusive byte scan, so empty pieces
+// are skipped). One thread per work item, a binary search over the scan: the
 // same work for one huge piece as for many small ones.
-__global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, int64_t n_tiles, uint32_t *fp) {
+__global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, uint32_t split_from, uint32_t split_bytes,
+                              int64_t n_tiles, uint32_t *fp) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_tiles) return;
-    const uint64_t b = (uint64_t)t * tb;
+    uint64_t b, e;
+    item_range(tb, split_from, split_bytes, ~0ull, (uint32_t)t, b, e);
     int64_t lo = 0, hi = n - 1;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
@@ -258,11 +259,11 @@ __global__ void k_first_piece(const uint64_t *pend, int64_t n, uint64_t tb, int6
     fp[t] = (uint32_t)lo;
 }
 
-hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, int64_t n_tiles,
-                              uint32_t *first_piece, hipStream_t stream) {
+hipError_t launch_first_piece(const uint64_t *pend, int64_t n, uint64_t tile_bytes, uint32_t split_from,
+                              uint32_t split_bytes, int64_t n_tiles, uint32_t *first_piece, hipStream_t stream) {
     if (n == 0 || n_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(k_first_piece, dim3((unsigned)((n_tiles + 255) / 256)), dim3(256), 0, stream, pend, n,
-                       tile_bytes, n_tiles, first_piece);
+                       tile_bytes, split_from, split_bytes, n_tiles, first_piece);
     return hipGetLastError();
 }
 

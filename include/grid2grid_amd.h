@@ -38,8 +38,10 @@ extern "C" {
  * g2g_batch_get/set_tuning, g2g_transform_plans_noalloc.
  * 3 changes g2g_tuning: stream_order takes the word version 2 reserved,
  * and row_chunks plus a reserved word are added before ms (the struct grows
- * from 40 to 48 bytes); adds G2G_ORDER_*. */
-#define G2G_ABI_VERSION 3
+ * from 40 to 48 bytes); adds G2G_ORDER_*.
+ * 4: g2g_tuning.tail_split takes the word version 3 reserved (same size;
+ * 0 keeps version 3's work items). */
+#define G2G_ABI_VERSION 4
 
 /* Element types; the reference instantiates transform for these four
  * (src/grid2grid/transform.cpp:206-298). */
@@ -94,7 +96,7 @@ typedef struct g2g_plan_info {
     int64_t bytes;           /* matrix bytes moved (m*n*sizeof(T)) */
     int64_t remote_bytes;    /* bytes whose source and destination ranks differ */
     int64_t n_tiles;         /* work items of the fused gather */
-    int64_t tile_bytes;      /* bytes per work item */
+    int64_t tile_bytes;      /* bytes per work item (the last round shorter with a tail split) */
     int32_t n_ranks;         /* max(P_initial, P_final) */
     int32_t dtype;
     int32_t trans;           /* 'N', 'T' or 'C' */
@@ -244,9 +246,9 @@ int g2g_batch_execute_graph(g2g_batch batch, void *stream);
 struct g2g_tuning;
 int g2g_batch_autotune(g2g_batch batch, void *stream, int reps, struct g2g_tuning *best);
 
-/* Launch configuration of a batch (column_chunks, stream_order and
- * row_chunks are 0: each plan runs its own stream) and setting one (those
- * three must be 0): launch_mode G2G_LAUNCH_PERSISTENT or
+/* Launch configuration of a batch (column_chunks, stream_order, row_chunks
+ * and tail_split are 0: each plan runs its own stream, cut into the batch's
+ * work items) and setting one (those four must be 0): launch_mode G2G_LAUNCH_PERSISTENT or
  * G2G_LAUNCH_PERSISTENT_XCD; a new work-item size re-cuts the batch's work
  * items (synchronous on `stream`). The captured graph, if any, is dropped. */
 int g2g_batch_get_tuning(g2g_batch batch, struct g2g_tuning *t);
@@ -266,7 +268,8 @@ int g2g_batch_destroy(g2g_batch batch);
  * GPU, held until the process ends) instead of allocations: the call
  * allocates nothing and copies nothing from the host (descriptors travel as
  * kernel arguments), so it can be captured. Each plan runs over its own work
- * items (its g2g_plan_get_tuning tile size). At most 4096 plans and 74401
+ * items (its g2g_plan_get_tuning tile size and tail split; the count is
+ * g2g_plan_get_info's n_tiles). At most 4096 plans and 73722
  * work items per call; larger requests fail with G2G_ERR_INVALID before
  * anything is launched. Every call on a GPU shares that GPU's array and
  * leaves its contents behind: calls that overlap on one GPU (other streams,
@@ -376,7 +379,11 @@ typedef struct g2g_tuning {
      * pieces move as bands of runs of h rows across many columns rather than
      * as long contiguous runs. Batches: 0. */
     int32_t row_chunks;
-    int32_t reserved;      /* 0 */
+    /* 0: every work item is tile_bytes long; d = 2, 4 or 8: the stream's
+     * last min(total / 2, CUs x 4 x tile_bytes) bytes (counted in whole items
+     * from the start) are cut into items of tile_bytes / d, so that the
+     * workgroups of a launch finish closer together. Batches: 0. */
+    int32_t tail_split;
     double ms;             /* measured time per transform (set by g2g_plan_autotune) */
 } g2g_tuning;
 
@@ -389,7 +396,8 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
  * on `stream`, hipEvents), time the eight fastest of each order again (each
  * also at the neighbouring work-item sizes) on the pieces and on
  * column-chunked (and, for tall pieces, row-chunked) streams in that order,
- * keep the fastest for the plan and record it for the plan's shape class
+ * time the fastest again as it is and with tail splits 2 and 4, keep the
+ * fastest for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform
  * runs while tuning (the final layout is written). *best receives the

@@ -205,7 +205,7 @@ extern "C" {
 // padding dropped in phase 2, 6 work items reaching one column into the next.
 int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype, char trans, int scaled,
                   int64_t tile_bytes, int threads, int feat, uint32_t chunk_w, int order, uint32_t chunk_h,
-                  int plant, char *msg, int msg_len,
+                  uint32_t split, int plant, char *msg, int msg_len,
                   int64_t *stats) {
     replay R;
     const uint32_t es = dtype == G2G_FLOAT32 ? 4 : dtype == G2G_COMPLEX128 ? 16 : 8;
@@ -257,13 +257,27 @@ int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype
     std::vector<uint64_t> pend(np);
     uint64_t total = 0;
     for (int64_t p = 0; p < np; ++p) pend[p] = total += (uint64_t)pieces[p].rows * pieces[p].cols * es;
+    // work items (retile): tb bytes each; with a tail split, the last
+    // min(total / 2, 256 CUs x 4 x tb) bytes after whole items in items of tb / split
     const uint64_t tb = (uint64_t)tile_bytes;
-    const uint64_t n_tiles = (total + tb - 1) / tb;
+    uint64_t full = (total + tb - 1) / tb, n_tiles = full, sb = tb;
+    if (split) {
+        sb = tb / split;
+        const uint64_t tail = std::min(total / 2, (uint64_t)256 * 4 * tb);
+        full = (total - tail) / tb;
+        n_tiles = full + (total - full * tb + sb - 1) / sb;
+    }
+    // first piece of every work item (k_first_piece): the first p with
+    // pend[p] > lo, found by walking the pieces alongside the items
     std::vector<uint32_t> first(n_tiles, 0);
-    for (int64_t p = 0; p < np; ++p) {  // k_first_piece
-        const uint64_t s = p ? pend[p - 1] : 0, e = pend[p];
-        if (e <= s) continue;
-        for (uint64_t t = (s + tb - 1) / tb; t * tb < e; ++t) first[t] = (uint32_t)p;
+    {
+        int64_t p = 0;
+        for (uint64_t t = 0; t < n_tiles; ++t) {
+            uint64_t lo, hi;
+            item_range(tb, (uint32_t)full, (uint32_t)sb, total, (uint32_t)t, lo, hi);
+            while (p < np - 1 && pend[p] <= lo) ++p;
+            first[t] = (uint32_t)p;
+        }
     }
     request rq;
     std::memset(&rq, 0, sizeof rq);
@@ -275,8 +289,16 @@ int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype
     std::vector<seg_entry> segs(kMaxSeg + 1), tsegs(kMaxSeg + 1);
     const uint32_t tile_areas = threads >= 512 ? trans_tiles<512>() : trans_tiles<256>();
 
+    uint64_t covered = 0;  // the items' ranges must tile the stream in order
     for (uint64_t t = 0; t < n_tiles; ++t) {
-        const uint64_t lo = t * tb, hi = std::min(lo + tb, total);
+        uint64_t lo, hi;
+        item_range(tb, (uint32_t)full, (uint32_t)sb, total, (uint32_t)t, lo, hi);
+        if (lo != covered || hi <= lo) {
+            std::snprintf(msg, msg_len, "work item %llu covers [%llu, %llu) after %llu", (unsigned long long)t,
+                          (unsigned long long)lo, (unsigned long long)hi, (unsigned long long)covered);
+            return 1;
+        }
+        covered = hi;
         const uint32_t n = (t + 1 < n_tiles ? first[t + 1] : (uint32_t)np - 1) + 1;  // last_piece + 1
         uint32_t p = first[t];
         for (;;) {
@@ -515,6 +537,8 @@ int replay_gather(const g2g_layout *initial, const g2g_layout *final_, int dtype
             p += 64;
         }
     }
+    if (covered != total)
+        R.error("work items cover %llu of %llu bytes", (unsigned long long)covered, (unsigned long long)total);
     // every final element written
     int64_t unwritten = 0;
     for (size_t k = 0; k < R.fin.key.size(); ++k)

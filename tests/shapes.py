@@ -80,7 +80,7 @@ def random_stream_cases(seed=4242, count=36):
     irregular arbitrary layouts, every type, 'N'/'T'/'C') with random
     tunings: (init, final, dtype, trans, tuning), tuning = (tile_bytes,
     threads, unroll_bytes, nontemporal, launch_mode, column_chunks,
-    stream_order, row_chunks)."""
+    stream_order, row_chunks, tail_split)."""
     from grid2grid_amd import workloads
     from grid2grid_amd import _lib
     dtypes = [_lib.FLOAT32, _lib.FLOAT64, _lib.COMPLEX64, _lib.COMPLEX128]
@@ -107,5 +107,6 @@ def random_stream_cases(seed=4242, count=36):
         w = int(rng.choice([0, 1, 2, 3, 8, 16, 64]))
         order = int(rng.integers(0, 2))
         h = int(rng.choice([0, 0, 1, 5, 16, 128])) if trans == "N" else 0
-        cases.append((init, fin, dtype, trans, (tile, th, ub, nt, mode, w, order, h)))
+        sp = int(rng.choice([0, 0, 2, 4, 8]))
+        cases.append((init, fin, dtype, trans, (tile, th, ub, nt, mode, w, order, h, sp)))
     return cases

@@ -38,7 +38,7 @@ def test_version():
     """The library reports the ABI version its header declares."""
     text = open(os.path.join(os.path.dirname(__file__), "..", "include", "grid2grid_amd.h")).read()
     want = int(re.search(r"#define G2G_ABI_VERSION (\d+)", text).group(1))
-    assert _lib.load().g2g_version() == want == 3
+    assert _lib.load().g2g_version() == want == 4
 
 
 def _plan_err(init, fin, dtype=g2g.FLOAT64, trans="N"):

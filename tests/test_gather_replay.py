@@ -26,13 +26,13 @@ DTYPES = [g2g.FLOAT32, g2g.FLOAT64, g2g.COMPLEX64, g2g.COMPLEX128]
 _lib = None
 
 
-def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant, order=0, chunk_h=0):
+def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, plant, order=0, chunk_h=0, split=0):
     global _lib
     if _lib is None:
         _lib = ctypes.CDLL(LIB)
         _lib.replay_gather.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_char, ctypes.c_int,
                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
-                                       ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int64)]
     if feat is None:
         feat = FEAT_ALL if scaled else (FEAT_TRANS if trans != "N" else FEAT_COPY)
@@ -43,17 +43,19 @@ def _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, pl
     msg = ctypes.create_string_buffer(4096)
     stats = (ctypes.c_int64 * 6)()
     errors = _lib.replay_gather(ctypes.addressof(di.struct), ctypes.addressof(df.struct), dtype, trans.encode(),
-                                int(scaled), tile_bytes, threads, feat, chunk_w, order, chunk_h, plant, msg, len(msg),
+                                int(scaled), tile_bytes, threads, feat, chunk_w, order, chunk_h, split, plant, msg,
+                                len(msg),
                                 stats)
     return errors, msg.value.decode(), list(stats)
 
 
 def _replay(init, fin, dtype, trans="N", scaled=False, tile_bytes=65536, threads=512, feat=None, chunk_w=0,
-            order=0, chunk_h=0):
-    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0, order, chunk_h)
+            order=0, chunk_h=0, split=0):
+    errors, msg, stats = _run(init, fin, dtype, trans, scaled, tile_bytes, threads, feat, chunk_w, 0, order, chunk_h,
+                              split)
     assert errors == 0, (f"{errors} errors ({init.m}x{init.n} dtype {dtype} trans {trans} scaled {scaled} "
                          f"tile {tile_bytes} threads {threads} feat {feat} chunks {chunk_w} order {order} "
-                         f"rows {chunk_h}):\n" + msg)
+                         f"rows {chunk_h} split {split}):\n" + msg)
     return stats
 
 
@@ -74,6 +76,8 @@ def test_replay_untransposed(name):
             _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
         for w, h, o in ((0, 3, 0), (8, 16, 1), (3, 7, 1)):
             _replay(init, fin, dtype, tile_bytes=12288, threads=512, chunk_w=w, order=o, chunk_h=h)
+        for sp, w, o in ((2, 0, 0), (4, 8, 1), (8, 0, 1)):
+            _replay(init, fin, dtype, tile_bytes=12288, threads=256, chunk_w=w, order=o, split=sp)
 
 
 @pytest.mark.parametrize("name", list(shapes.transposed_pairs()))
@@ -90,6 +94,8 @@ def test_replay_transposed(name):
                 _replay(init, fin, dtype, trans, tile_bytes=65536, threads=512, chunk_w=w)
             for w, tb in ((0, 49152), (q, 40960)):
                 _replay(init, fin, dtype, trans, tile_bytes=tb, threads=512, chunk_w=w, order=1)
+            for sp in (2, 8):
+                _replay(init, fin, dtype, trans, tile_bytes=65536, threads=512, split=sp)
 
 
 def test_replay_cut_edges_take_both_paths():
@@ -115,12 +121,16 @@ def test_replay_ragged_columns(dtype):
 
 def test_replay_benchmark_block_cyclic_4096():
     """The 4096^2 f64 block-cyclic benchmark pair at full size (16 384
-    pieces), on the pieces and on chunks of 16 columns."""
+    pieces), on the pieces and on chunks of 16 columns, and with the last
+    quarter of its work items (256 CUs x 4 items of 32 KiB) split."""
     init, fin, dtype, trans = workloads.bc4096()
     st = _replay(init, fin, dtype, tile_bytes=65536, threads=512)
     assert st[0] == 2048
     _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=16)
     _replay(init, fin, dtype, tile_bytes=49152, threads=512, chunk_w=2, order=1)
+    for sp in (2, 4):
+        st = _replay(init, fin, dtype, tile_bytes=32768, threads=256, chunk_w=64, split=sp)
+        assert st[0] == 3072 + 1024 * sp
 
 
 def test_replay_mixed_requests():
@@ -148,18 +158,20 @@ def test_replay_irregular_small(dtype):
         _replay(init, fin, dtype, tile_bytes=tb, threads=512, chunk_w=w, order=1)
     for w, h in ((8, 256), (32, 128)):
         _replay(init, fin, dtype, tile_bytes=65536, threads=512, chunk_w=w, order=1, chunk_h=h)
+    _replay(init, fin, dtype, tile_bytes=40960, threads=512, chunk_w=2, order=1, split=2)
 
 
 def test_replay_random_streams():
     """The random requests and tunings of the GPU test
     test_random_requests_random_streams, replayed: every work item, queue
     slot and transpose tile of each stream (column and row chunks, either
-    order, any work-item size) checked against the transform's definition."""
+    order, any work-item size, split tails) checked against the transform's
+    definition."""
     feats = {"N": FEAT_COPY, "T": FEAT_TRANS, "C": FEAT_TRANS}
     for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases()):
-        tile, th, ub, nt, mode, w, order, h = tu
+        tile, th, ub, nt, mode, w, order, h, sp = tu
         _replay(init, fin, dtype, trans, tile_bytes=tile, threads=th, feat=feats[trans], chunk_w=w, order=order,
-                chunk_h=h)
+                chunk_h=h, split=sp)
 
 
 @pytest.mark.parametrize("plant,case", [(1, "ragged"), (2, "ragged"), (3, "cut"), (4, "cut"), (5, "cut"), (6, "arb"),

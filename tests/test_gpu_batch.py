@@ -1,12 +1,14 @@
 """GPU: many transforms of mixed types and operations in one persistent
 launch (batches, hipGraph replay) and through the serving loop; every result
 must equal the oracle's result for that transform alone."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import oracle_lib as O
 import grid2grid_amd as g2g
-from grid2grid_amd import workloads
+from grid2grid_amd import _lib, workloads
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -103,7 +105,8 @@ def test_batch_launch_modes_and_autotune(launch_mode):
 def test_batch_set_tuning_every_variant():
     """Every settable batch configuration (work-item size, workgroup size,
     bytes in flight, non-temporal hint, both persistent modes) gives every
-    request's own result; settings outside the compiled variants are refused."""
+    request's own result; settings outside the compiled variants, and stream
+    fields other than 0, are refused."""
     cases = _cases(12, 9, scaled_every=4)
     batch = g2g.Batch([c.plan for c in cases], [c.alpha for c in cases], [c.beta for c in cases])
     for tb, th, ub, nt, mode in [(4096, 256, 64, 0, 1), (32768, 512, 128, 1, 3), (1 << 20, 512, 64, 0, 3),
@@ -123,6 +126,12 @@ def test_batch_set_tuning_every_variant():
                 dict(tile_bytes=32768, unroll_bytes=96)]:
         with pytest.raises(g2g.G2GError):
             batch.set_tuning(**bad)
+    # the stream fields of a batch's own tuning must be 0 (each plan runs its
+    # own stream, cut into the batch's work items)
+    for k, field in enumerate(("column_chunks", "stream_order", "row_chunks", "tail_split")):
+        t = _lib.G2GTuning(65536, 512, 128, 0, 1, 0, 0, 0, 0, 0.0)
+        setattr(t, field, 2)
+        assert _lib.load().g2g_batch_set_tuning(batch.handle, ctypes.byref(t), None) != 0, field
     batch.close()
 
 

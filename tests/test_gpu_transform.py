@@ -286,38 +286,42 @@ def test_column_chunked_stream(name):
     ordered down each column interval) and over the pieces or chunks in
     destination order writes exactly what the pieces in their own order
     write, for odd widths, work-item sizes that are not powers of two and
-    every launch mode, also cut into row chunks; the width, order and row
-    chunk height are reported back; negative widths or heights and an
-    unknown order are refused."""
+    every launch mode, also cut into row chunks and with the last round of
+    work items split; the width, order, row chunk height and tail split are
+    reported back; negative widths or heights, an unknown order and a tail
+    split other than 0, 2, 4 or 8 are refused."""
     init, fin = _layout_pairs()[name]
     dtype = g2g.FLOAT32
     rng = np.random.default_rng(17)
     ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
     want = _expected(init, ia, fin, fa, dtype)
     ti = _dev(ia)
-    for w, mode, tile, order, h in ((1, 0, 4096, 0, 0), (3, 1, 65536, 1, 0), (16, 2, 4096, 0, 0),
-                                    (64, 3, 1 << 20, 1, 0), (3, 0, 1 << 20, 0, 0), (0, 1, 40960, 1, 0),
-                                    (0, 0, 4096, 1, 0), (0, 3, 49152, 1, 0), (16, 1, 12288, 1, 0), (0, 1, 4096, 0, 5),
-                                    (8, 0, 12288, 1, 16), (3, 3, 65536, 1, 3), (32, 1, 8192, 0, 64)):
-        print(f"chunks {name} w {w} mode {mode} tile {tile} order {order} rows {h}", flush=True)
+    for w, mode, tile, order, h, sp in ((1, 0, 4096, 0, 0, 0), (3, 1, 65536, 1, 0, 2), (16, 2, 4096, 0, 0, 4),
+                                        (64, 3, 1 << 20, 1, 0, 0), (3, 0, 1 << 20, 0, 0, 8), (0, 1, 40960, 1, 0, 0),
+                                        (0, 0, 4096, 1, 0, 2), (0, 3, 49152, 1, 0, 4), (16, 1, 12288, 1, 0, 0),
+                                        (0, 1, 4096, 0, 5, 8), (8, 0, 12288, 1, 16, 0), (3, 3, 65536, 1, 3, 4),
+                                        (32, 1, 8192, 0, 64, 2)):
+        print(f"chunks {name} w {w} mode {mode} tile {tile} order {order} rows {h} split {sp}", flush=True)
         tf = _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype)
-        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        plan.set_tuning(tile, 512, 128, 0, mode, column_chunks=w, stream_order=order, row_chunks=h, tail_split=sp)
         t = plan.tuning
         assert t["column_chunks"] == w and t["stream_order"] == order and t["row_chunks"] == h
+        assert t["tail_split"] == sp
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
-        # back to the pieces in their own order
+        # back to the pieces in their own order, whole work items
         tf.copy_(_dev(fa))
         plan.set_tuning(tile, 256, 64, 1, mode, column_chunks=0)
         t = plan.tuning
-        assert t["column_chunks"] == 0 and t["stream_order"] == 0 and t["row_chunks"] == 0
+        assert t["column_chunks"] == 0 and t["stream_order"] == 0 and t["row_chunks"] == 0 and t["tail_split"] == 0
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want)
         plan.close()
-    for bad in ({"column_chunks": -1}, {"stream_order": 2}, {"stream_order": -1}, {"row_chunks": -1}):
+    for bad in ({"column_chunks": -1}, {"stream_order": 2}, {"stream_order": -1}, {"row_chunks": -1},
+                {"tail_split": 3}, {"tail_split": -2}, {"tail_split": 16}):
         with pytest.raises(g2g.G2GError):
             plan = g2g.Plan(init, ti.data_ptr(), fin, ti.data_ptr(), dtype)
             try:
@@ -360,7 +364,8 @@ def test_tall_pieces_autotune_row_chunks_and_shape_class():
     """Tall pieces (4 KiB columns: 512-row f64 blocks, and one block per side):
     the autotuner's pick, which may cut them into row bands, gives the oracle's
     bytes; a plan of the same shape created afterwards starts from the same
-    stream (column chunks, row chunks, order) and gives them too."""
+    stream (column chunks, row chunks, order) and work items (size, tail
+    split) and gives them too."""
     dtype = g2g.FLOAT64
     rng = np.random.default_rng(61)
     for init, fin in ((scalapack_layout(2048, 96, 512, 32, 2, 1, "C"), scalapack_layout(2048, 96, 1024, 48, 2, 2, "C")),
@@ -379,7 +384,7 @@ def test_tall_pieces_autotune_row_chunks_and_shape_class():
         ti2, tf2 = _dev(ia), _dev(fa)
         plan2 = g2g.Plan(init, ti2.data_ptr(), fin, tf2.data_ptr(), dtype)
         t2 = plan2.tuning
-        for k in ("column_chunks", "row_chunks", "stream_order", "tile_bytes"):
+        for k in ("column_chunks", "row_chunks", "stream_order", "tile_bytes", "tail_split"):
             assert t2[k] == best[k], (k, t2, best)
         plan2.transform()
         torch.cuda.synchronize()
@@ -498,13 +503,13 @@ def test_random_requests_random_streams():
     rng = np.random.default_rng(4343)
     cases = []
     for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases()):
-        tile, th, ub, nt, mode, w, order, h = tu
+        tile, th, ub, nt, mode, w, order, h, sp = tu
         ia, fa = _rand_arena(init, dtype, rng), _rand_arena(fin, dtype, rng)
         print(f"case {k}: {init.m}x{init.n} dtype {dtype} trans {trans} tuning {tu}", flush=True)
         want = _expected(init, ia, fin, fa, dtype, trans)
         ti, tf = _dev(ia), _dev(fa)
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
-        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h, tail_split=sp)
         plan.transform()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tf.cpu().numpy(), want, err_msg=f"case {k}")

@@ -96,7 +96,7 @@ def test_noalloc_refuses_beyond_capacity():
         big = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), g2g.FLOAT32)
     finally:
         g2g.set_tuning(0, -1)
-    assert big.info.n_tiles > 74401
+    assert big.info.n_tiles > 73722
     with pytest.raises(g2g.G2GError) as e:
         g2g.transform_plans_noalloc([big])
     assert e.value.code == 1 and "work items" in str(e.value)
@@ -114,7 +114,7 @@ def test_noalloc_tuned_streams_two_calls():
     rng = np.random.default_rng(77)
     all_cases = []
     for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases(seed=5151, count=18)):
-        tile, th, ub, nt, mode, w, order, h = tu
+        tile, th, ub, nt, mode, w, order, h, sp = tu
         es = g2g.DTYPE_SIZE[dtype]
         ia = rng.integers(0, 256, size=init.arena_elems * es, dtype=np.uint8)
         fa = rng.integers(0, 256, size=fin.arena_elems * es, dtype=np.uint8)
@@ -122,7 +122,7 @@ def test_noalloc_tuned_streams_two_calls():
         O.transform(init, ia.copy(), fin, want, dtype, trans, None, None)
         ti, tf = torch.from_numpy(ia).cuda(), torch.from_numpy(fa).cuda()
         plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
-        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h)
+        plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h, tail_split=sp)
         all_cases.append(dict(plan=plan, ti=ti, tf=tf, fa=fa, want=want, alpha=None, beta=None,
                               size=init.m * init.n * es, tuning=tu))
     by_size = sorted(all_cases, key=lambda c: c["size"])

@@ -103,15 +103,24 @@ WORKLOAD_DESC = {
 }
 
 
-def time_steps(torch, fn, steps, warmup, stream, per_launch=False):
+def time_steps(torch, fn, steps, warmup, stream, per_launch=False, warm_s=0.0):
     """Average ms per step over `steps` back-to-back steps (events only at the
-    two ends of the timed region). With per_launch, a second, separate pass
-    brackets every step with its own event pair on the launch stream and
-    returns those durations too (their spread; each also holds the dispatch
-    latency between the first event and the kernel's start)."""
+    two ends of the timed region), after `warmup` untimed ones, and, with
+    warm_s, more untimed ones until the warm-up has lasted warm_s seconds (a
+    burst of back-to-back launches starts slower for about 15 ms). With
+    per_launch, a second, separate pass brackets every step with its own event
+    pair on the launch stream and returns those durations too (their spread;
+    each also holds the dispatch latency between the first event and the
+    kernel's start)."""
+    t0 = time.perf_counter()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    spent = time.perf_counter() - t0
+    if warmup and spent < warm_s:
+        for _ in range(int((warm_s - spent) / (spent / warmup)) + 1):
+            fn()
+        torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record(stream)
@@ -192,7 +201,7 @@ def workload_leg(torch, g2g, workloads, name, stream, dev, steps, warmup, rotate
 
     n = max(steps, rotate)
     n -= n % rotate  # every pair the same number of times
-    ms, per = time_steps(torch, step, n, max(warmup, rotate), stream, per_launch=True)
+    ms, per = time_steps(torch, step, n, max(warmup, rotate), stream, per_launch=True, warm_s=0.03)
     bad = sum(g2g.verify_layout(fin, tf.data_ptr(), dtype, 100 + k, trans, init.n, stream)
               for k, (_, tf, _) in enumerate(pairs))
     nbytes = workloads.algorithmic_bytes(fin, dtype)

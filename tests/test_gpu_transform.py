@@ -572,15 +572,25 @@ def test_stream_probe():
             g2g.stream_probe(bad[0], src.data_ptr(), dst.data_ptr(), bad[1])
 
 
-@pytest.mark.parametrize("name,chunks", [("bc4096", 0), ("bc32768", 0), ("bc32768", 16), ("irregular", 0),
-                                         ("irregular", 8)])
-def test_benchmark_workloads_full_size(name, chunks):
+# (tile_bytes or None for the plan's own, threads, unroll_bytes, nontemporal,
+# launch_mode, column_chunks, stream_order, row_chunks, tail_split): the
+# pieces, plain column chunks, and the configurations the autotuner picks for
+# these workloads (destination order, tail split)
+FULL_SIZE_CASES = [("bc4096", (None, 512, 64, 1, 1, 0, 0, 0, 0)), ("bc4096", (32768, 256, 128, 1, 0, 64, 0, 0, 2)),
+                   ("bc32768", (None, 512, 64, 1, 1, 0, 0, 0, 0)), ("bc32768", (None, 512, 64, 1, 1, 16, 0, 0, 0)),
+                   ("bc32768", (65536, 512, 64, 1, 1, 32, 1, 0, 0)),
+                   ("irregular", (None, 512, 64, 1, 1, 0, 0, 0, 0)), ("irregular", (None, 512, 64, 1, 1, 8, 0, 0, 0)),
+                   ("irregular", (40960, 512, 128, 1, 1, 2, 1, 0, 0))]
+
+
+@pytest.mark.parametrize("name,tu", FULL_SIZE_CASES)
+def test_benchmark_workloads_full_size(name, tu):
     """The benchmark's own layout pairs at full size: 4096^2 f64, 32768^2 f64
     (8 GiB a side) and the irregular 40000x25000 f32 with padded, shuffled
-    blocks, on the pieces and on a column-chunked stream. Every element of the
-    final layout equals the seeded value of its source element (checked on the
-    device), and no byte outside the final blocks (ld padding, gaps) is
-    touched."""
+    blocks, on the pieces, on column-chunked streams and in the configurations
+    the autotuner picks for them. Every element of the final layout equals the
+    seeded value of its source element (checked on the device), and no byte
+    outside the final blocks (ld padding, gaps) is touched."""
     from grid2grid_amd import workloads
     init, fin, dtype, trans = getattr(workloads, name)()
     es = g2g.DTYPE_SIZE[dtype]
@@ -588,9 +598,10 @@ def test_benchmark_workloads_full_size(name, chunks):
     tf = torch.full((fin.arena_elems * es,), 0xA5, dtype=torch.uint8, device="cuda")
     g2g.fill_layout(init, ti.data_ptr(), dtype, 11)
     plan = g2g.Plan(init, ti.data_ptr(), fin, tf.data_ptr(), dtype, trans)
-    t = plan.tuning
-    plan.set_tuning(t["tile_bytes"], 512, 64, 1, 1, column_chunks=chunks)
-    print(f"{name} chunks {chunks}: {plan.info.n_pieces} pieces, {plan.info.n_tiles} work items", flush=True)
+    tile, th, ub, nt, mode, w, order, h, sp = tu
+    plan.set_tuning(tile or plan.tuning["tile_bytes"], th, ub, nt, mode, column_chunks=w, stream_order=order,
+                    row_chunks=h, tail_split=sp)
+    print(f"{name} tuning {tu}: {plan.info.n_pieces} pieces, {plan.info.n_tiles} work items", flush=True)
     plan.transform()
     torch.cuda.synchronize()
     assert g2g.verify_layout(fin, tf.data_ptr(), dtype, 11, trans, init.n) == 0

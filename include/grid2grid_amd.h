@@ -396,8 +396,8 @@ int g2g_plan_set_tuning(g2g_plan plan, const g2g_tuning *t);
  * on `stream`, hipEvents), time the eight fastest of each order again (each
  * also at the neighbouring work-item sizes) on the pieces and on
  * column-chunked (and, for tall pieces, row-chunked) streams in that order,
- * time the fastest again as it is and with tail splits 2 and 4, keep the
- * fastest for the plan and record it for the plan's shape class
+ * time the fastest again as it is and with tail splits 2 and 4 (a split is
+ * taken when it is 0.5 % faster), keep the result for the plan and record it for the plan's shape class
  * (element size, operation, size class, piece-size class): plans created
  * later in this process with the same class start from it. The transform
  * runs while tuning (the final layout is written). *best receives the

@@ -66,9 +66,9 @@ def ragged_pair(dtype):
 
 def irregular_small():
     """The irregular benchmark pair's construction at 6000 x 400: row blocks
-    of 40-520 rows (f32 columns of 160 B to 2 KB, so some below and many
-    above the phased-column size), ld = rows + U[0, 63], blocks shuffled
-    with gaps, so columns start at every dword offset of a 128-byte line."""
+    of 40-520 rows (f32 columns of 160 B to 2 KB), ld = rows + U[0, 63],
+    blocks shuffled with gaps, so columns start at every dword offset of a
+    128-byte line."""
     rng = np.random.default_rng(303)
     init = arbitrary_layout(rng, 6000, 400, 16, 5, 6, lo=40, hi=520, pad=(0, 63), gap=(0, 64))
     fin = arbitrary_layout(rng, 6000, 400, 13, 4, 5, lo=40, hi=520, pad=(0, 63), gap=(0, 64))

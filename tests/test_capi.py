@@ -133,3 +133,15 @@ def test_layout_descriptor_is_kept_per_address():
     np.testing.assert_array_equal(b.blocks["data"] - a.blocks["data"], np.uint64(1 << 20))
     assert lay.descriptor(1 << 20, 4) is not a  # the element size is part of the address arithmetic
 
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without the built HIP library nothing falls back to a CPU path: loading
+    it, and with it creating a plan, raises."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", os.path.join(os.path.dirname(_lib.LIB_PATH), "absent.so"))
+    with pytest.raises(ImportError):
+        _lib.load()
+    lay = scalapack_layout(8, 8, 4, 4, 1, 1, "C")
+    with pytest.raises(ImportError):
+        g2g.Plan(lay, 0, lay, 0, g2g.FLOAT64)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+import shapes
 import grid2grid_amd as g2g
 from grid2grid_amd import _lib, workloads
 
@@ -172,6 +173,27 @@ def test_server_batched_results_equal_alone():
     assert all(x >= 0 for x in lat)
     for k, c in enumerate(cases):
         c.check(f"served request {k}")
+
+
+def test_server_tuned_plans():
+    """Plans on tuned streams (column and row chunks, destination order, tail
+    splits, any work-item size), served batched: each result is the
+    oracle's; the scaled ones run over the plans' pieces."""
+    rng = np.random.default_rng(29)
+    cases = []
+    for k, (init, fin, dtype, trans, tu) in enumerate(shapes.random_stream_cases(seed=6161, count=24)):
+        c = Case(init, fin, dtype, trans, rng, scaled=k % 6 == 0)
+        tile, th, ub, nt, mode, w, order, h, sp = tu
+        c.plan.set_tuning(tile, th, ub, nt, mode, column_chunks=w, stream_order=order, row_chunks=h, tail_split=sp)
+        cases.append(c)
+    srv = g2g.Server(1 << 40, n_streams=2, max_batch=6)
+    tickets = [srv.submit(c.plan, c.alpha, c.beta) for c in cases]
+    for t in tickets:
+        srv.wait(t)
+    srv.drain()
+    srv.close()
+    for k, c in enumerate(cases):
+        c.check(f"served tuned request {k}")
 
 
 def test_server_reports_each_ticket_once():

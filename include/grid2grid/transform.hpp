@@ -128,6 +128,15 @@ class transform_plan {
         detail::check(g2g_plan_autotune(p_, stream, reps, &t));
         return t;
     }
+    // The launch configuration (g2g_plan_get_tuning / g2g_plan_set_tuning):
+    // work-item size, kernel variant, stream (column chunks, row chunks,
+    // order) and tail split.
+    g2g_tuning tuning() const {
+        g2g_tuning t;
+        detail::check(g2g_plan_get_tuning(p_, &t));
+        return t;
+    }
+    void set_tuning(const g2g_tuning &t) { detail::check(g2g_plan_set_tuning(p_, &t)); }
     g2g_plan handle() const { return p_; }
 
   private:

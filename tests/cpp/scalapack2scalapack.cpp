@@ -114,6 +114,23 @@ static bool run_case(int m, int n, int ibm, int ibn, int fbm, int fbn, int pm, i
         reset();
         plan.execute();
         verify();
+        // another configuration set by hand: 3-column chunks in destination
+        // order, the last round of work items halved
+        g2g_tuning u = plan.tuning();
+        u.tile_bytes = 12288;
+        u.column_chunks = 3;
+        u.stream_order = G2G_ORDER_DESTINATION;
+        u.row_chunks = 0;
+        u.tail_split = 2;
+        plan.set_tuning(u);
+        const g2g_tuning v = plan.tuning();
+        if (v.column_chunks != 3 || v.stream_order != G2G_ORDER_DESTINATION || v.tail_split != 2) {
+            std::printf("set_tuning not reported back\n");
+            ok = false;
+        }
+        reset();
+        plan.execute();
+        verify();
         reset();
     }
     int64_t hits0 = 0;

@@ -220,11 +220,12 @@ typedef struct g2g_batch_s *g2g_batch;
  * their single-T restriction. alphas[k]/betas[k] point to one element of plan
  * k's dtype, or the arrays are NULL (plain copies). Plans must outlive the
  * batch. The batch runs each plan's piece stream as it was at creation:
- * once a plan's stream changes (column chunks switched on or off or to
- * another width, or the stream order changed, by g2g_plan_set_tuning or
- * g2g_plan_autotune) the batch's
- * launches fail with G2G_ERR_INVALID until it is built again; a new work-item
- * size or kernel variant of the plan leaves the batch as it is. */
+ * once a plan's stream changes (column or row chunks switched on or off or
+ * to another width or height, or the stream order changed, by
+ * g2g_plan_set_tuning or g2g_plan_autotune) the batch's launches fail with
+ * G2G_ERR_INVALID until it is built again; a new work-item size, tail split
+ * or kernel variant of the plan leaves the batch as it is (a batch cuts the
+ * plans' streams into its own work items). */
 int g2g_batch_create(int n, const g2g_plan *plans, const void *const *alphas,
                      const void *const *betas, void *stream, g2g_batch *out);
 
